@@ -1,0 +1,84 @@
+"""Builds artful_amd/libartful_hip.so: the host C library (scene, fluid, renderer, HIP backend)
+with gcc, the HIP kernels with hipcc for gfx950, linked by hipcc.
+
+Host C is built with the reference's floating-point flags (-ffp-contract=off -fno-math-errno
+-fno-trapping-math, Artful CMakeLists.txt:46-58), since the disc map, the fluid and the frame
+uniforms must come out bit for bit as the reference's. The kernel files choose their own
+contraction: trace_exact.hip and post.hip none (the reference's roundings), trace_fast.hip fused.
+
+    python -m artful_amd.build        (or artful_amd.build.build())
+"""
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OUT = PKG / "libartful_hip.so"
+OBJ = ROOT / "_build" / "artful_hip"
+ARCH = os.environ.get("ARTF_OFFLOAD_ARCH", "gfx950")
+
+C_SOURCES = ["core/base.c", "core/pool.c", "scene/disc.c", "scene/fluid.c", "scene/hole.c", "scene/scene.c",
+             "render/renderer.c", "render/hip_backend.c"]
+# (source, contraction)
+HIP_SOURCES = [("hip/trace_exact.hip", "off"), ("hip/trace_fast.hip", "fast"), ("hip/post.hip", "off"),
+               ("hip/abi.hip", "off")]
+
+C_FLAGS = ["-std=c11", "-O3", "-fPIC", "-Wall", "-Wno-parentheses", "-ffp-contract=off", "-fno-math-errno",
+           "-fno-trapping-math", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and Path(c).exists():
+            return c
+    raise RuntimeError("hipcc not found: the HIP kernels cannot be built")
+
+
+def hip_flags(contract: str) -> list[str]:
+    return [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", f"-ffp-contract={contract}",
+            "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-command-line-argument", f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    noise = [ln for ln in r.stdout.splitlines() if "packed-fp32-ops" not in ln and ln.strip()]
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + "\n".join(noise))
+    if noise:
+        print("\n".join(noise), file=sys.stderr)
+
+
+def build(verbose: bool = False) -> Path:
+    OBJ.mkdir(parents=True, exist_ok=True)
+    jobs = []
+    objs = []
+    for src in C_SOURCES:
+        o = OBJ / (src.replace("/", "_") + ".o")
+        jobs.append(["gcc", *C_FLAGS, "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(o)
+    cc = hipcc()
+    for src, contract in HIP_SOURCES:
+        o = OBJ / (src.replace("/", "_") + ".o")
+        jobs.append([cc, *hip_flags(contract), "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(o)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        for cmd, fut in zip(jobs, [ex.submit(_run, j) for j in jobs]):
+            if verbose:
+                print(" ".join(cmd))
+            fut.result()
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread", "-lm"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,121 @@
+/* artful-hip — the C ABI over the kernels (include/artful_hip.h). Every entry checks what it is given
+ * on the host before anything is launched, and returns a HIP status; nothing here allocates except
+ * artfHipMalloc. */
+#include <stdio.h>
+#include <string.h>
+
+#include "launch.h"
+
+using namespace artf;
+
+extern "C" {
+
+int artfHipDeviceCount(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int artfHipSetDevice(int device) { return (int)hipSetDevice(device); }
+
+int artfHipDeviceInfo(int device, char* name, size_t size, int* computeUnits) {
+    hipDeviceProp_t p;
+    const hipError_t e = hipGetDeviceProperties(&p, device);
+    if (e != hipSuccess) return (int)e;
+    if (name && size) snprintf(name, size, "%s (%s)", p.name, p.gcnArchName);
+    if (computeUnits) *computeUnits = p.multiProcessorCount;
+    return 0;
+}
+
+void* artfHipMalloc(size_t bytes) {
+    void* p = NULL;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return NULL;
+    return p;
+}
+
+void artfHipFree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int artfHipUpload(void* dst, const void* src, size_t bytes, void* stream) {
+    return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+}
+
+int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+    if (e != hipSuccess) return (int)e;
+    return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+int artfHipMemset(void* dst, int value, size_t bytes, void* stream) {
+    return (int)hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
+}
+
+void* artfHipStreamCreate(void) {
+    hipStream_t s = NULL;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NULL;
+    return (void*)s;
+}
+
+void artfHipStreamDestroy(void* stream) {
+    if (stream) (void)hipStreamDestroy((hipStream_t)stream);
+}
+
+int artfHipStreamSync(void* stream) { return (int)hipStreamSynchronize((hipStream_t)stream); }
+
+void* artfHipEventCreate(void) {
+    hipEvent_t e = NULL;
+    if (hipEventCreate(&e) != hipSuccess) return NULL;
+    return (void*)e;
+}
+
+void artfHipEventDestroy(void* event) {
+    if (event) (void)hipEventDestroy((hipEvent_t)event);
+}
+
+int artfHipEventRecord(void* event, void* stream) { return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream); }
+
+float artfHipEventMs(void* start, void* stop) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop) != hipSuccess) return -1.f;
+    return ms;
+}
+
+const char* artfHipErrorText(int status) { return hipGetErrorString((hipError_t)status); }
+
+static bool frameFits(const ArtfHoleFrame* f) {
+    return f && f->size[0] > 0 && f->size[1] > 0 && f->size[2] >= 1 && f->size[2] <= 8 && f->size[3] >= 1 && f->size[3] <= ARTF_BLOOM_LEVELS &&
+           (uint64_t)f->size[0] * f->size[1] * f->size[2] * f->size[2] < (1ull << 31);
+}
+
+int artfHipTrace(const ArtfHipTraceArgs* a, void* stream) {
+    if (!a || !frameFits(&a->frame) || !a->map || !a->fluid || !a->out[0] || !a->out[1] || !a->out[2]) return (int)hipErrorInvalidValue;
+    if (a->maxSteps < 0 || a->maxSteps > 1000000 || a->rowBegin < 0) return (int)hipErrorInvalidValue;
+    if (a->precision == ARTF_PRECISION_FP32) return (int)launchTraceExact(*a, (hipStream_t)stream);
+    return (int)launchTraceFast(*a, (hipStream_t)stream);
+}
+
+int artfHipResolve(const ArtfHoleFrame* f, float* const rays[3], float* const hdr[3], void* stream) {
+    if (!frameFits(f) || !rays || !hdr) return (int)hipErrorInvalidValue;
+    return (int)launchResolve(*f, rays, hdr, (hipStream_t)stream);
+}
+
+int artfHipBloom(const ArtfHoleFrame* f, float* const hdr[3], float* const bloom[3], void* stream) {
+    if (!frameFits(f) || !hdr || !bloom) return (int)hipErrorInvalidValue;
+    /* the level table must be the one artfHoleFrame builds: each level half the last, packed */
+    uint32_t w = f->size[0], h = f->size[1], at = 0;
+    for (uint32_t k = 0; k < f->size[3]; ++k) {
+        w = (w + 1) / 2, h = (h + 1) / 2;
+        if (f->bloom[k][0] != at || f->bloom[k][1] != w || f->bloom[k][2] != h) return (int)hipErrorInvalidValue;
+        at += w * h;
+    }
+    return (int)launchBloom(*f, hdr, bloom, (hipStream_t)stream);
+}
+
+int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, void* stream) {
+    if (!frameFits(f) || !hdr || !bloom || !bgra) return (int)hipErrorInvalidValue;
+    if (f->bloom[0][1] != (f->size[0] + 1) / 2 || f->bloom[0][2] != (f->size[1] + 1) / 2) return (int)hipErrorInvalidValue;
+    return (int)launchToneMap(*f, hdr, bloom, bgra, (hipStream_t)stream);
+}
+
+} // extern "C"

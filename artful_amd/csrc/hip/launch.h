@@ -1,0 +1,24 @@
+/* artful-hip — what the kernel files hand each other: launchers, one per translation unit, so that
+ * each unit can be built with its own floating-point contraction rule. */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "artful_hip.h"
+
+namespace artf {
+
+/* The tile a workgroup traces: 16 x 16 pixels as four waves of 8 x 8, for coherent neighbours. */
+constexpr int kTile = 16;
+constexpr int kGroup = 256;
+
+/* FP32, the reference's arithmetic bit for bit (trace_exact.hip, contraction off). */
+hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
+/* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */
+hipError_t launchTraceFast(const ArtfHipTraceArgs& a, hipStream_t stream);
+/* Resolve, bloom and tone map (post.hip, contraction off). */
+hipError_t launchResolve(const ArtfHoleFrame& f, float* const rays[3], float* const hdr[3], hipStream_t stream);
+hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], hipStream_t stream);
+hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, hipStream_t stream);
+
+} // namespace artf

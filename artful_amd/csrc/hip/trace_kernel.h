@@ -1,0 +1,104 @@
+/* artful-hip — the tracing kernel: a workgroup of 256 lanes traces a 16 x 16 tile of pixels (four
+ * waves of 8 x 8, so neighbouring rays, which take similar paths, share a wave) for one of the
+ * pixel's side^2 rays (grid z). Included by trace_exact.hip and trace_fast.hip, which differ only in
+ * their contraction rule. */
+#pragma once
+
+#include "launch.h"
+#include "mixed.h"
+#include "tracer.h"
+
+namespace artf {
+
+/* A ray's place in its pixel: the reference's stratified grid (lens.inc:367), or with a seed, a
+ * jittered point in the same stratum. */
+__device__ __forceinline__ void subPixel(uint32_t seed, int px, int py, int sx, int sy, int side, float& dx, float& dy) {
+    if (!seed) {
+        dx = ((float)sx + .5f) / (float)side;
+        dy = ((float)sy + .5f) / (float)side;
+        return;
+    }
+    const uint32_t h = lensHash((uint32_t)px ^ (seed * 0x9E3779B1u), (uint32_t)py, (uint32_t)(sy * side + sx) + seed);
+    const float jx = ((float)(h & 0xFFFFu) + .5f) * (1.f / 65536.f), jy = ((float)(h >> 16) + .5f) * (1.f / 65536.f);
+    dx = ((float)sx + jx) / (float)side;
+    dy = ((float)sy + jy) / (float)side;
+}
+
+/* The wave's step count and its lane-slots (its longest ray times its lanes with a pixel), added to
+ * two counters by one lane. */
+__device__ __forceinline__ void countSteps(unsigned long long* counters, int steps, bool valid) {
+    if (!counters) return;
+    unsigned long long sum = (unsigned long long)steps;
+    int most = steps, lanes = valid ? 1 : 0;
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        sum += __shfl_xor(sum, offset);
+        most = max(most, __shfl_xor(most, offset));
+        lanes += __shfl_xor(lanes, offset);
+    }
+    if ((threadIdx.x & 63) == 0 && lanes) {
+        atomicAdd(&counters[0], sum);
+        atomicAdd(&counters[1], (unsigned long long)most * (unsigned long long)lanes);
+    }
+}
+
+template <int Precision>
+__global__ __launch_bounds__(kGroup) void traceKernel(ArtfHipTraceArgs a) {
+    const ArtfHoleFrame& f = a.frame;
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+    const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+    const int py = a.rowBegin + blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+    const int sub = blockIdx.z, sx = sub % side, sy = sub / side;
+    const bool valid = px < width && py < rowEnd;
+
+    TraceLaw law;
+    law.maxSteps = a.maxSteps > 0 ? a.maxSteps : 192;
+    law.uEscape = a.escapeRadius > 0.f ? 1.f / a.escapeRadius : 0.f;
+    law.disc = a.disc;
+    law.q = a.emissivity;
+
+    RayResult ray = {0.f, 0.f, 0.f, 0};
+    if (valid) {
+        float dx, dy;
+        subPixel(a.jitter, px, py, sx, sy, side, dx, dy);
+        const float moment = ((float)((sub * 5) % (side * side)) + .5f) * f.image[3] - .5f;
+        const float time = f.disc[2] + f.spin[3] * moment;
+        if (Precision == ARTF_PRECISION_FP64) {
+            Eye<double> e;
+            eyeFrom64(a.frame64, e);
+            const double nx = ((double)px + (double)dx) * (2.0 / (double)width) - 1.0;
+            const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
+            ray = traceRay<double>(f, e, law, a.map, a.fluid, nx, ny, time);
+        } else if (Precision == ARTF_PRECISION_MIXED) {
+            const double nx = ((double)px + (double)dx) * (2.0 / (double)width) - 1.0;
+            const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
+            ray = traceRayMixed(f, a.frame64, law, a.map, a.fluid, nx, ny, time);
+        } else {
+            Eye<float> e;
+            eyeFrom<float>(f, e);
+            /* (as lensTile forms it: the pixel's group of 8, plus the offset, then the lane) */
+            const float nx = ((float)(px & 7) + ((float)(px & ~7) + dx)) * f.image[1] - 1.f;
+            const float ny = 1.f - ((float)py + dy) * f.image[2];
+            ray = traceRay<float>(f, e, law, a.map, a.fluid, nx, ny, time);
+        }
+    }
+    countSteps(a.counters, ray.steps, valid);
+    if (!valid) return;
+    const size_t at = (side == 1 ? 0 : (size_t)sub * width * height) + (size_t)py * width + px;
+    a.out[0][at] = ray.r;
+    a.out[1][at] = ray.g;
+    a.out[2][at] = ray.b;
+}
+
+inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHipTraceArgs& a, hipStream_t stream) {
+    const int width = (int)a.frame.size[0], height = (int)a.frame.size[1], side = (int)a.frame.size[2];
+    const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
+    const int rows = rowEnd - a.rowBegin;
+    if (width <= 0 || rows <= 0 || a.rowBegin < 0 || side < 1 || side > 8) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((rows + kTile - 1) / kTile), (unsigned)(side * side));
+    hipLaunchKernelGGL(kernel, grid, dim3(kGroup), 0, stream, a);
+    return hipGetLastError();
+}
+
+} // namespace artf

@@ -1,0 +1,355 @@
+/* artful-hip — the HIP backend: a renderer that draws a scene's black hole on an AMD Instinct GPU.
+ *
+ * The drop-in beside Artful 0.4.0's software and Vulkan backends (src/software/software.c:762-789,
+ * src/vulkan/vulkan.c:1691-1783, 2566-2584): it fills the same vtable, reads the same scene data (the
+ * map, uploaded when mapVersion changes; the fluid field, when fluidVersion changes; the frame's
+ * uniforms from artfHoleFrame), and per frame runs trace -> (resolve) -> bloom -> tone map on one
+ * stream, timed with events between the stages. It is plain C over include/artful_hip.h.
+ *
+ * Device memory per renderer: the map (11.5 MB) and fluid (1 MB) once; per size, the HDR planes
+ * (12 B/px), the rays' planes when a pixel has several (12 B/ray), the bloom pyramid, and BGRA. */
+#include "renderer_internal.h"
+
+#include <stdio.h>
+
+#include "../scene/scene_internal.h"
+#include "artful_hip.h"
+
+int artfHipCallsLeft = -1;
+
+enum { EV_START, EV_TRACED, EV_RESOLVED, EV_BLOOMED, EV_TONED, EV_COUNT };
+
+typedef struct HipRenderer {
+    ArtfRenderer base;
+    int device;
+    void* stream;
+    void* events[EV_COUNT];
+    char description[256];
+    int width, height;
+    ArtfHoleTracing tracing;
+    bool lost;
+    /* scene data on the device, and which version of which scene it is */
+    float* map;
+    uint64_t mapScene, mapVersion;
+    float* fluid;
+    uint64_t fluidScene, fluidVersion;
+    /* per-size targets */
+    float* hdr[3];
+    size_t hdrPixels;
+    float* rays[3];
+    size_t rayCount;
+    float* bloom[3];
+    size_t bloomPixels;
+    uint32_t* bgra;
+    size_t bgraPixels;
+    unsigned long long* counters;
+    /* the last frame */
+    bool drawn, hole;
+    uint32_t background;
+    ArtfHoleFrame frame;
+    ArtfHoleStats holeStats;
+    double cpuMs;
+    /* the overlay, copied */
+    uint32_t* overlay;
+    int overlayWidth, overlayHeight;
+} HipRenderer;
+
+/* A device call's status: false (and the renderer lost) on failure, or when the test hook says so. */
+static bool ok(HipRenderer* h, int status, const char* what) {
+    if (status == 0 && artfHipCallsLeft != 0) {
+        if (artfHipCallsLeft > 0) --artfHipCallsLeft;
+        return true;
+    }
+    if (status) artfLog("artful: HIP %s failed: %s", what, artfHipErrorText(status));
+    else artfLog("artful: HIP %s: the device is lost (test hook)", what);
+    h->lost = true;
+    return false;
+}
+
+static void freePlanes(float** p) {
+    for (int c = 0; c < 3; ++c) {
+        artfHipFree(p[c]);
+        p[c] = NULL;
+    }
+}
+
+static bool allocPlanes(HipRenderer* h, float** p, size_t floats) {
+    for (int c = 0; c < 3; ++c) {
+        p[c] = artfHipMalloc(floats * sizeof(float));
+        if (!p[c]) {
+            artfLog("artful: HIP: out of device memory for %zu floats", floats);
+            freePlanes(p);
+            h->lost = true;
+            return false;
+        }
+    }
+    return true;
+}
+
+static const char* hipDescription(const ArtfRenderer* r) { return ((const HipRenderer*)r)->description; }
+
+static void hipResize(ArtfRenderer* r, int width, int height) {
+    HipRenderer* h = (HipRenderer*)r;
+    h->width = artfMaxi(width, 1);
+    h->height = artfMaxi(height, 1);
+    h->drawn = false;
+}
+
+static void hipSetOverlay(ArtfRenderer* r, const ArtfOverlay* o) {
+    HipRenderer* h = (HipRenderer*)r;
+    if (!o || !o->pixels) {
+        artfFree(h->overlay);
+        h->overlay = NULL;
+        return;
+    }
+    const int stride = o->stride ? o->stride : o->width;
+    if (!h->overlay || h->overlayWidth != o->width || h->overlayHeight != o->height) {
+        artfFree(h->overlay);
+        h->overlay = artfMalloc((size_t)o->width * o->height * 4);
+        h->overlayWidth = o->width;
+        h->overlayHeight = o->height;
+    }
+    for (int y = 0; y < o->height; ++y) memcpy(h->overlay + (size_t)y * o->width, o->pixels + (size_t)y * stride, (size_t)o->width * 4);
+}
+
+static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
+    const size_t pixels = (size_t)h->width * h->height;
+    if (pixels != h->hdrPixels) {
+        freePlanes(h->hdr);
+        artfHipFree(h->bgra);
+        h->bgra = NULL;
+        h->hdrPixels = h->bgraPixels = 0;
+        if (!allocPlanes(h, h->hdr, pixels)) return false;
+        h->bgra = artfHipMalloc(pixels * 4);
+        if (!h->bgra) return ok(h, 2, "allocate");
+        h->hdrPixels = h->bgraPixels = pixels;
+    }
+    if (rays > pixels && rays > h->rayCount) {
+        freePlanes(h->rays);
+        h->rayCount = 0;
+        if (!allocPlanes(h, h->rays, rays)) return false;
+        h->rayCount = rays;
+    }
+    if (bloomPixels > h->bloomPixels) {
+        freePlanes(h->bloom);
+        h->bloomPixels = 0;
+        if (!allocPlanes(h, h->bloom, bloomPixels)) return false;
+        h->bloomPixels = bloomPixels;
+    }
+    if (!h->counters) {
+        h->counters = artfHipMalloc(2 * sizeof(unsigned long long));
+        if (!h->counters) return ok(h, 2, "allocate");
+    }
+    return true;
+}
+
+/* The map and the fluid on the device, as the scene now has them. */
+static bool syncScene(HipRenderer* h, const ArtfScene* s) {
+    const ArtfHole* hole = &s->hole;
+    if (!h->map) {
+        h->map = artfHipMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
+        h->fluid = artfHipMalloc((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float));
+        if (!h->map || !h->fluid) return ok(h, 2, "allocate");
+    }
+    if (h->mapScene != s->id || h->mapVersion != hole->mapVersion) {
+        if (!ok(h, artfHipUpload(h->map, hole->map, (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float), h->stream), "map upload")) return false;
+        h->mapScene = s->id;
+        h->mapVersion = hole->mapVersion;
+    }
+    if (h->fluidScene != s->id || h->fluidVersion != hole->fluidVersion) {
+        if (!ok(h, artfHipUpload(h->fluid, artfFluidField(hole->fluid), (size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float), h->stream), "fluid upload"))
+            return false;
+        h->fluidScene = s->id;
+        h->fluidVersion = hole->fluidVersion;
+    }
+    return true;
+}
+
+static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
+    HipRenderer* h = (HipRenderer*)r;
+    if (h->lost) return false;
+    const double t0 = artfNowMs();
+    h->drawn = false;
+    h->hole = scene->hole.on;
+    memset(&h->holeStats, 0, sizeof h->holeStats);
+    if (!h->hole) {
+        /* the scene's background, as the reference's empty frame (software.c:597) */
+        h->background = 0xFF000000u | 13u << 16 | 15u << 8 | 20u;
+        h->drawn = true;
+        h->cpuMs = artfNowMs() - t0;
+        return true;
+    }
+    uint32_t bloomPixels = 0;
+    artfHoleFrameLevels(scene, h->width, h->height, h->tracing.bloomLevels, &h->frame, &bloomPixels);
+    const int side = (int)h->frame.size[2];
+    const size_t rays = (size_t)h->width * h->height * side * side;
+    if (!ensure(h, bloomPixels, rays)) return false;
+    const double tUp = artfNowMs();
+    if (!syncScene(h, scene)) return false;
+
+    ArtfHipTraceArgs a;
+    memset(&a, 0, sizeof a);
+    a.frame = h->frame;
+    if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a.frame64);
+    a.map = h->map;
+    a.fluid = h->fluid;
+    float** out = side == 1 ? h->hdr : h->rays;
+    for (int c = 0; c < 3; ++c) a.out[c] = out[c];
+    a.counters = h->counters;
+    a.precision = h->tracing.precision;
+    a.maxSteps = h->tracing.maxSteps;
+    a.escapeRadius = h->tracing.escapeRadius;
+    a.disc = h->tracing.disc;
+    a.emissivity = h->tracing.emissivity;
+    a.jitter = h->tracing.jitter;
+
+    if (!ok(h, artfHipMemset(h->counters, 0, 2 * sizeof(unsigned long long), h->stream), "clear")) return false;
+    if (!ok(h, artfHipEventRecord(h->events[EV_START], h->stream), "event")) return false;
+    if (!ok(h, artfHipTrace(&a, h->stream), "trace")) return false;
+    if (!ok(h, artfHipEventRecord(h->events[EV_TRACED], h->stream), "event")) return false;
+    if (side > 1 && !ok(h, artfHipResolve(&h->frame, h->rays, h->hdr, h->stream), "resolve")) return false;
+    if (!ok(h, artfHipEventRecord(h->events[EV_RESOLVED], h->stream), "event")) return false;
+    if (!ok(h, artfHipBloom(&h->frame, h->hdr, h->bloom, h->stream), "bloom")) return false;
+    if (!ok(h, artfHipEventRecord(h->events[EV_BLOOMED], h->stream), "event")) return false;
+    if (!ok(h, artfHipToneMap(&h->frame, h->hdr, h->bloom, h->bgra, h->stream), "tone map")) return false;
+    if (!ok(h, artfHipEventRecord(h->events[EV_TONED], h->stream), "event")) return false;
+    unsigned long long counts[2] = {0, 0};
+    if (!ok(h, artfHipDownload(counts, h->counters, sizeof counts, h->stream), "wait")) return false;
+
+    ArtfHoleStats* st = &h->holeStats;
+    st->traceMs = artfHipEventMs(h->events[EV_START], h->events[EV_TRACED]);
+    st->resolveMs = artfHipEventMs(h->events[EV_TRACED], h->events[EV_RESOLVED]);
+    st->bloomMs = artfHipEventMs(h->events[EV_RESOLVED], h->events[EV_BLOOMED]);
+    st->toneMs = artfHipEventMs(h->events[EV_BLOOMED], h->events[EV_TONED]);
+    st->uploadMs = 0;
+    (void)tUp;
+    st->rays = rays;
+    st->steps = counts[0];
+    st->laneSlots = counts[1];
+    st->bloomLevels = (int)h->frame.size[3];
+    h->drawn = true;
+    h->cpuMs = artfNowMs() - t0;
+    return true;
+}
+
+/* Premultiplied overlay over the frame, per channel: o + f (255 - oa) / 255, rounded. */
+static void compose(const HipRenderer* h, uint32_t* dst) {
+    if (!h->overlay) return;
+    const int w = artfMini(h->overlayWidth, h->width), hh = artfMini(h->overlayHeight, h->height);
+    for (int y = 0; y < hh; ++y)
+        for (int x = 0; x < w; ++x) {
+            const uint32_t o = h->overlay[(size_t)y * h->overlayWidth + x], oa = o >> 24;
+            if (!oa) continue;
+            uint32_t* p = dst + (size_t)y * h->width + x;
+            if (oa == 255) {
+                *p = o;
+                continue;
+            }
+            uint32_t out = 0;
+            for (int c = 0; c < 32; c += 8) {
+                const uint32_t v = ((o >> c) & 255) + (((*p >> c) & 255) * (255 - oa) + 127) / 255;
+                out |= (v > 255 ? 255 : v) << c;
+            }
+            *p = out;
+        }
+}
+
+static bool hipReadPixels(ArtfRenderer* r, uint32_t* dst) {
+    HipRenderer* h = (HipRenderer*)r;
+    if (h->lost || !h->drawn) return false;
+    const size_t pixels = (size_t)h->width * h->height;
+    if (!h->hole) {
+        for (size_t i = 0; i < pixels; ++i) dst[i] = h->background;
+    } else if (!ok(h, artfHipDownload(dst, h->bgra, pixels * 4, h->stream), "read back")) {
+        return false;
+    }
+    compose(h, dst);
+    return true;
+}
+
+static bool hipReadHdr(ArtfRenderer* r, float* red, float* green, float* blue) {
+    HipRenderer* h = (HipRenderer*)r;
+    if (h->lost || !h->drawn || !h->hole) return false;
+    const size_t bytes = (size_t)h->width * h->height * sizeof(float);
+    float* dst[3] = {red, green, blue};
+    for (int c = 0; c < 3; ++c)
+        if (!ok(h, artfHipDownload(dst[c], h->hdr[c], bytes, h->stream), "read back")) return false;
+    return true;
+}
+
+static uint32_t* hipPixels(ArtfRenderer* r) {
+    (void)r;
+    return NULL;
+}
+
+static void hipStats(const ArtfRenderer* r, ArtfFrameStats* out) {
+    const HipRenderer* h = (const HipRenderer*)r;
+    const ArtfHoleStats* st = &h->holeStats;
+    out->cpuMs = h->cpuMs;
+    out->lensMs = st->traceMs + st->resolveMs + st->bloomMs;
+    out->rasterMs = st->toneMs;
+    out->gpuMs = out->lensMs + out->rasterMs;
+}
+
+static bool hipHoleStats(const ArtfRenderer* r, ArtfHoleStats* out) {
+    const HipRenderer* h = (const HipRenderer*)r;
+    *out = h->holeStats;
+    return h->drawn && h->hole;
+}
+
+static void hipSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* t) {
+    HipRenderer* h = (HipRenderer*)r;
+    h->tracing = t ? *t : artfHoleTracingDefaults();
+    h->drawn = false;
+}
+
+static void hipDestroy(ArtfRenderer* r) {
+    HipRenderer* h = (HipRenderer*)r;
+    if (h->stream) artfHipStreamSync(h->stream);
+    freePlanes(h->hdr);
+    freePlanes(h->rays);
+    freePlanes(h->bloom);
+    artfHipFree(h->bgra);
+    artfHipFree(h->counters);
+    artfHipFree(h->map);
+    artfHipFree(h->fluid);
+    for (int i = 0; i < EV_COUNT; ++i) artfHipEventDestroy(h->events[i]);
+    artfHipStreamDestroy(h->stream);
+    artfFree(h->overlay);
+    artfFree(h);
+}
+
+static const ArtfRendererVtbl hipTable = {hipDescription, hipResize, hipSetOverlay, hipDraw,    hipReadPixels,     hipPixels,
+                                          hipStats,       hipDestroy, hipReadHdr,    hipSetHoleTracing, hipHoleStats};
+
+ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
+    const int count = artfHipDeviceCount();
+    if (count <= 0 || o->device < 0 || o->device >= count) {
+        artfLog("artful: no HIP device %d here (%d found)", o->device, count);
+        return NULL;
+    }
+    if (artfHipSetDevice(o->device)) {
+        artfLog("artful: HIP device %d cannot be used", o->device);
+        return NULL;
+    }
+    HipRenderer* h = ARTF_NEW(HipRenderer);
+    h->base.vt = &hipTable;
+    h->base.backend = ARTF_BACKEND_HIP;
+    h->device = o->device;
+    h->stream = artfHipStreamCreate();
+    for (int i = 0; i < EV_COUNT; ++i) h->events[i] = artfHipEventCreate();
+    bool ready = h->stream != NULL;
+    for (int i = 0; i < EV_COUNT; ++i) ready = ready && h->events[i];
+    if (!ready) {
+        artfLog("artful: HIP device %d: no stream or events", o->device);
+        hipDestroy(&h->base);
+        return NULL;
+    }
+    char name[160] = "";
+    int units = 0;
+    artfHipDeviceInfo(o->device, name, sizeof name, &units);
+    snprintf(h->description, sizeof h->description, "HIP, %s, %d compute units", name, units);
+    h->tracing = artfHoleTracingDefaults();
+    hipResize(&h->base, o->width, o->height);
+    return &h->base;
+}

@@ -1,0 +1,52 @@
+/* artful-hip — a scene: a camera and, perhaps, a black hole (Artful 0.4.0 src/scene/scene.c:27-78, 941,
+ * as far as the hole's path goes). */
+#include "scene_internal.h"
+
+#include <stdatomic.h>
+
+#include "artful/inspect.h"
+
+ArtfScene* artfSceneNew(void) {
+    static atomic_ullong lastId;
+    ArtfScene* s = ARTF_NEW(ArtfScene);
+    s->id = atomic_fetch_add(&lastId, 1) + 1;
+    s->camera.fovY = 1.f;
+    s->camera.near_ = .1f;
+    s->camera.up = artfVec3(0, 1, 0);
+    s->camera.position = artfVec3(0, 0, 1);
+    return s;
+}
+
+void artfSceneFree(ArtfScene* s) {
+    if (!s) return;
+    artfFree(s->hole.map);
+    artfFluidFree(s->hole.fluid);
+    artfFree(s);
+}
+
+void artfSceneSetCamera(ArtfScene* s, const ArtfCamera* camera) { s->camera = *camera; }
+
+bool artfSceneHoleDesc(const ArtfScene* s, ArtfBlackHoleDesc* out) {
+    if (!s->hole.on) return false;
+    *out = s->hole.desc;
+    return true;
+}
+
+const float* artfSceneHoleMap(const ArtfScene* s, uint64_t* version) {
+    if (version) *version = s->hole.mapVersion;
+    return s->hole.map;
+}
+
+const float* artfSceneHoleFluid(const ArtfScene* s, uint64_t* version, double* time) {
+    if (version) *version = s->hole.fluidVersion;
+    if (time) *time = s->hole.fluid ? artfFluidTime(s->hole.fluid) : 0.0;
+    return s->hole.fluid ? artfFluidField(s->hole.fluid) : NULL;
+}
+
+uint32_t artfSceneHoleFrame(const ArtfScene* s, int width, int height, int levels, ArtfHoleFrame* out) {
+    uint32_t bloomPixels = 0;
+    artfHoleFrameLevels(s, width, height, levels, out, &bloomPixels);
+    return bloomPixels;
+}
+
+void artfSceneHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out) { artfHoleFrame64(s, width, height, out); }

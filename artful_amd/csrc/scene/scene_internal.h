@@ -1,0 +1,47 @@
+/* artful-hip — the scene's insides, as the HIP backend reads them. */
+#ifndef ARTFUL_SCENE_INTERNAL_H
+#define ARTFUL_SCENE_INTERNAL_H
+
+#include "artful/hole.h"
+#include "artful/scene.h"
+
+typedef struct ArtfFluid ArtfFluid;
+
+/* The hole of a scene: its clamped description, its tables and its fluid. map and fluid change
+ * only with their versions, which is what a backend compares to know when to upload them again. */
+typedef struct ArtfHole {
+    bool on;
+    ArtfBlackHoleDesc desc;
+    float* map;            /* ARTF_HOLE_MAP_FLOATS */
+    uint64_t mapVersion;
+    ArtfFluid* fluid;
+    uint64_t fluidVersion;
+} ArtfHole;
+
+struct ArtfScene {
+    uint64_t id;
+    ArtfCamera camera;
+    ArtfHole hole;
+};
+
+/* ---- disc.c: the painted gas and the per-ring tables (Artful src/scene/hole.c:10-197) ---- */
+void artfDiscPaint(float* map);  /* gas, dust, colours, sky: once per map */
+void artfDiscRings(float* map, float inner, float outer, float spin);
+
+/* ---- fluid.c (Artful src/scene/fluid.c) ---- */
+ArtfFluid* artfFluidNew(float inner, float outer, float spin, double time);
+void artfFluidFree(ArtfFluid* f);
+bool artfFluidAdvance(ArtfFluid* f, double time);
+const float* artfFluidField(const ArtfFluid* f);
+double artfFluidTime(const ArtfFluid* f);
+
+/* ---- hole.c ---- */
+float artfHoleIsco(float spin);
+/* The frame's uniforms (Artful src/scene/hole.c:273-371); bloomPixels: what the bloom's levels take.
+ * levelsCap: 0 for the reference's rule, else at most that many bloom levels. */
+void artfHoleFrame(const ArtfScene* s, int width, int height, ArtfHoleFrame* out, uint32_t* bloomPixels);
+void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCap, ArtfHoleFrame* out, uint32_t* bloomPixels);
+/* The same geometry from the same description, worked out in doubles. */
+void artfHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out);
+
+#endif

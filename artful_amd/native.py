@@ -1,0 +1,246 @@
+"""Python binding of libartful_hip.so (the C library: scene, fluid, HIP renderer, kernel C ABI).
+
+A thin ctypes layer for tests, the bench and scripts; the library itself is C and HIP. Loading fails
+loudly when the library has not been built (python -m artful_amd.build): there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libartful_hip.so"
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built: run python -m artful_amd.build")
+    L = C.CDLL(str(LIB_PATH))
+    P, I, F, D, U32, U64, B = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_uint32, C.c_uint64, C.c_bool
+    sig = {
+        "artfSceneNew": ([], P), "artfSceneFree": ([P], None),
+        "artfSceneSetCamera": ([P, C.POINTER(abi.Camera)], None),
+        "artfSceneSetBlackHole": ([P, C.POINTER(abi.BlackHoleDesc)], None),
+        "artfBlackHoleDefaults": ([], abi.BlackHoleDesc), "artfBlackHoleOrbitSpeed": ([F, F], F),
+        "artfHoleIsco": ([F], F),
+        "artfSceneHoleDesc": ([P, C.POINTER(abi.BlackHoleDesc)], B),
+        "artfSceneHoleMap": ([P, C.POINTER(U64)], C.POINTER(C.c_float)),
+        "artfSceneHoleFluid": ([P, C.POINTER(U64), C.POINTER(D)], C.POINTER(C.c_float)),
+        "artfSceneHoleFrame": ([P, I, I, I, C.POINTER(abi.HoleFrame)], U32),
+        "artfSceneHoleFrame64": ([P, I, I, C.POINTER(abi.HoleFrame64)], None),
+        "artfFluidNew": ([F, F, F, D], P), "artfFluidFree": ([P], None), "artfFluidAdvance": ([P, D], B),
+        "artfFluidField": ([P], C.POINTER(C.c_float)), "artfFluidTime": ([P], D),
+        "artfRendererDefaults": ([I, I], abi.RendererOptions),
+        "artfRendererNew": ([C.POINTER(abi.RendererOptions)], P), "artfRendererFree": ([P], None),
+        "artfRendererBackend": ([P], I), "artfRendererDescription": ([P], C.c_char_p),
+        "artfRendererResize": ([P, I, I], None), "artfRendererSetOverlay": ([P, C.POINTER(abi.Overlay)], None),
+        "artfRendererDraw": ([P, P], B), "artfRendererReadPixels": ([P, P], B), "artfRendererPixels": ([P], P),
+        "artfRendererStats": ([P, C.POINTER(abi.FrameStats)], None),
+        "artfRendererReadHdr": ([P, P, P, P], B),
+        "artfHoleTracingDefaults": ([], abi.HoleTracing),
+        "artfRendererSetHoleTracing": ([P, C.POINTER(abi.HoleTracing)], None),
+        "artfRendererHoleStats": ([P, C.POINTER(abi.HoleStats)], B),
+        "artfBackendName": ([I], C.c_char_p),
+        "artfHipDeviceCount": ([], I), "artfHipSetDevice": ([I], I),
+        "artfHipDeviceInfo": ([I, C.c_char_p, C.c_size_t, C.POINTER(I)], I),
+        "artfHipMalloc": ([C.c_size_t], P), "artfHipFree": ([P], None),
+        "artfHipUpload": ([P, P, C.c_size_t, P], I), "artfHipDownload": ([P, P, C.c_size_t, P], I),
+        "artfHipMemset": ([P, I, C.c_size_t, P], I),
+        "artfHipStreamCreate": ([], P), "artfHipStreamDestroy": ([P], None), "artfHipStreamSync": ([P], I),
+        "artfHipEventCreate": ([], P), "artfHipEventDestroy": ([P], None), "artfHipEventRecord": ([P, P], I),
+        "artfHipEventMs": ([P, P], F), "artfHipErrorText": ([I], C.c_char_p),
+        "artfHipTrace": ([C.POINTER(abi.HipTraceArgs), P], I),
+        "artfHipResolve": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),
+        "artfHipBloom": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),
+        "artfHipToneMap": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P, P], I),
+    }
+    for name, (args, ret) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ret
+    L.artfHipCallsLeft = C.c_int.in_dll(L, "artfHipCallsLeft")
+    _lib = L
+    return L
+
+
+def declared_symbols() -> list[str]:
+    """The functions include/*.h declares (for the export check)."""
+    import re
+    root = Path(__file__).resolve().parent.parent / "include"
+    names = set()
+    for h in list(root.glob("*.h")) + list(root.glob("artful/*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(artf\w+)\s*\(", text, flags=re.M):
+            if "ARTF_INLINE" in m.group(0) or "typedef" in m.group(0) or "#define" in m.group(0):
+                continue
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def defaults() -> abi.BlackHoleDesc:
+    return lib().artfBlackHoleDefaults()
+
+
+def camera(position, target=(0, 0, 0), up=(0, 1, 0), fov_y=1.0, near=0.1) -> abi.Camera:
+    return abi.Camera(abi.Vec3(*position), abi.Vec3(*target), abi.Vec3(*up), fov_y, near)
+
+
+def hole_desc(**kw) -> abi.BlackHoleDesc:
+    d = defaults()
+    for k, v in kw.items():
+        if k in ("position", "axis", "velocity"):
+            setattr(d, k, abi.Vec3(*v))
+        else:
+            setattr(d, k, v)
+    return d
+
+
+class Scene:
+    """A scene with a camera and a black hole (artfSceneNew ... artfSceneFree)."""
+
+    def __init__(self, desc: abi.BlackHoleDesc | None = None, cam: abi.Camera | None = None):
+        self._L = lib()
+        self.ptr = self._L.artfSceneNew()
+        if cam is not None:
+            self.set_camera(cam)
+        if desc is not None:
+            self.set_hole(desc)
+
+    def set_camera(self, cam: abi.Camera) -> None:
+        self._L.artfSceneSetCamera(self.ptr, C.byref(cam))
+
+    def set_hole(self, desc: abi.BlackHoleDesc | None) -> None:
+        self._L.artfSceneSetBlackHole(self.ptr, C.byref(desc) if desc is not None else None)
+
+    def desc(self) -> abi.BlackHoleDesc | None:
+        d = abi.BlackHoleDesc()
+        return d if self._L.artfSceneHoleDesc(self.ptr, C.byref(d)) else None
+
+    def map(self) -> np.ndarray:
+        v = C.c_uint64()
+        p = self._L.artfSceneHoleMap(self.ptr, C.byref(v))
+        return np.ctypeslib.as_array(p, shape=(abi.MAP_FLOATS,)).copy()
+
+    def map_version(self) -> int:
+        v = C.c_uint64()
+        self._L.artfSceneHoleMap(self.ptr, C.byref(v))
+        return v.value
+
+    def fluid(self) -> tuple[np.ndarray, int, float]:
+        v, t = C.c_uint64(), C.c_double()
+        p = self._L.artfSceneHoleFluid(self.ptr, C.byref(v), C.byref(t))
+        return np.ctypeslib.as_array(p, shape=(abi.FLUID_WIDTH * abi.FLUID_HEIGHT,)).copy(), v.value, t.value
+
+    def frame(self, width: int, height: int, levels: int = 0) -> tuple[abi.HoleFrame, int]:
+        f = abi.HoleFrame()
+        n = self._L.artfSceneHoleFrame(self.ptr, width, height, levels, C.byref(f))
+        return f, n
+
+    def frame64(self, width: int, height: int) -> abi.HoleFrame64:
+        f = abi.HoleFrame64()
+        self._L.artfSceneHoleFrame64(self.ptr, width, height, C.byref(f))
+        return f
+
+    def close(self) -> None:
+        if self.ptr:
+            self._L.artfSceneFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return lib().artfHipDeviceCount()
+
+
+class Renderer:
+    """artfRendererNew with the HIP backend; raises when no GPU can run it."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        self._L = lib()
+        o = self._L.artfRendererDefaults(width, height)
+        o.backend = abi.BACKEND_HIP
+        o.device = device
+        self.ptr = self._L.artfRendererNew(C.byref(o))
+        if not self.ptr:
+            raise RuntimeError("artfRendererNew: no HIP device can run the renderer here")
+        self.width, self.height = width, height
+
+    def description(self) -> str:
+        return self._L.artfRendererDescription(self.ptr).decode()
+
+    def set_tracing(self, precision="fp32", disc="painted", emissivity=3.0, max_steps=0, escape_radius=0.0,
+                    jitter=0, bloom_levels=0) -> None:
+        t = self._L.artfHoleTracingDefaults()
+        t.precision = abi.PRECISION[precision] if isinstance(precision, str) else precision
+        t.disc = abi.DISC[disc] if isinstance(disc, str) else disc
+        t.emissivity, t.maxSteps, t.escapeRadius = emissivity, max_steps, escape_radius
+        t.jitter, t.bloomLevels = jitter, bloom_levels
+        self._L.artfRendererSetHoleTracing(self.ptr, C.byref(t))
+
+    def resize(self, width: int, height: int) -> None:
+        self._L.artfRendererResize(self.ptr, width, height)
+        self.width, self.height = width, height
+
+    def draw(self, scene: Scene) -> bool:
+        return bool(self._L.artfRendererDraw(self.ptr, scene.ptr))
+
+    def read_pixels(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width), np.uint32)
+        if not self._L.artfRendererReadPixels(self.ptr, out.ctypes.data):
+            raise RuntimeError("artfRendererReadPixels failed")
+        return out
+
+    def read_hdr(self) -> np.ndarray:
+        out = np.zeros((3, self.height, self.width), np.float32)
+        if not self._L.artfRendererReadHdr(self.ptr, out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data):
+            raise RuntimeError("artfRendererReadHdr failed")
+        return out
+
+    def stats(self) -> abi.FrameStats:
+        s = abi.FrameStats()
+        self._L.artfRendererStats(self.ptr, C.byref(s))
+        return s
+
+    def hole_stats(self) -> abi.HoleStats:
+        s = abi.HoleStats()
+        self._L.artfRendererHoleStats(self.ptr, C.byref(s))
+        return s
+
+    def set_overlay(self, pixels: np.ndarray | None) -> None:
+        if pixels is None:
+            self._L.artfRendererSetOverlay(self.ptr, None)
+            return
+        self._overlay = np.ascontiguousarray(pixels, np.uint32)
+        o = abi.Overlay(self._overlay.ctypes.data, self._overlay.shape[1], self._overlay.shape[0], 0, False, None, 0)
+        self._L.artfRendererSetOverlay(self.ptr, C.byref(o))
+
+    def close(self) -> None:
+        if self.ptr:
+            self._L.artfRendererFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def calls_left(n: int) -> None:
+    """Test hook: device calls that succeed before the device is reported lost (-1: no limit)."""
+    lib().artfHipCallsLeft.value = n

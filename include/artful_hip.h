@@ -1,0 +1,87 @@
+/* artful-hip — the thin C ABI over the HIP kernels (gfx950). Plain C: POD arguments, device pointers
+ * as plain pointers, streams as void*, an int status (0, else a HIP error number) and nothing thrown.
+ * The C host backend (artful_amd/csrc/render/hip_backend.c) calls these; so may any FFI (ctypes,
+ * cgo, JNI) that wants the kernels without the renderer.
+ *
+ * Each entry replaces a piece of Artful 0.4.0's black-hole path, named beside it. */
+#ifndef ARTFUL_HIP_H
+#define ARTFUL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "artful/hole.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- device, memory, streams, events (what src/vulkan/vulkan.c does through Vulkan) ---- */
+
+int artfHipDeviceCount(void);
+int artfHipSetDevice(int device);
+/* The device's name and compute units into name (size bytes); 0 on success. */
+int artfHipDeviceInfo(int device, char* name, size_t size, int* computeUnits);
+void* artfHipMalloc(size_t bytes); /* NULL when the device has no room */
+void artfHipFree(void* p);
+int artfHipUpload(void* dst, const void* src, size_t bytes, void* stream);
+int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream);
+int artfHipMemset(void* dst, int value, size_t bytes, void* stream);
+void* artfHipStreamCreate(void);
+void artfHipStreamDestroy(void* stream);
+int artfHipStreamSync(void* stream);
+void* artfHipEventCreate(void);
+void artfHipEventDestroy(void* event);
+int artfHipEventRecord(void* event, void* stream);
+/* Milliseconds between two recorded events (both must have completed). */
+float artfHipEventMs(void* start, void* stop);
+/* The last error the runtime saw, as text. */
+const char* artfHipErrorText(int status);
+
+/* ---- the tracer: replaces lensTile + lensTrace (src/software/lens.inc:99-383), which the
+ * reference calls per 64x64 tile from drawHole (src/software/software.c:410-418), and the Vulkan
+ * shader artfShaderHoleTrace (src/vulkan/shaders.c:1957-2026) dispatched by traceHole
+ * (src/vulkan/vulkan.c:1758-1783). ---- */
+
+typedef struct ArtfHipTraceArgs {
+    ArtfHoleFrame frame;     /* from artfHoleFrame: FP32 uniforms */
+    ArtfHoleFrame64 frame64; /* from artfHoleFrame64: read by the MIXED and FP64 tracers */
+    const float* map;        /* device, ARTF_HOLE_MAP_FLOATS */
+    const float* fluid;      /* device, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT */
+    /* device planes R, G, B. With one ray a pixel: the HDR image, width * height each. With side^2 rays
+     * a pixel: each ray's light, side^2 * width * height each, ray k of pixel p at k * width * height + p
+     * (artfHipResolve then averages them in the reference's order). */
+    float* out[3];
+    /* device, 2 words, added to: the rays' steps, and the lane-slots their waves spent stepping (so
+     * steps / slots is the stepping loop's active-lane fraction). NULL: not counted. */
+    unsigned long long* counters;
+    int precision;      /* ArtfPrecision (artful/renderer.h) */
+    int maxSteps;       /* 0: 192 */
+    float escapeRadius; /* 0: where 1/r crosses 0 */
+    int disc;           /* ArtfDiscModel */
+    float emissivity;   /* the analytic disc's q */
+    uint32_t jitter;    /* 0: the stratified grid; else a seed */
+    int rowBegin, rowEnd; /* rows [rowBegin, rowEnd) of the image; rowEnd 0: to the last row */
+} ArtfHipTraceArgs;
+
+int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
+
+/* The mean of each pixel's side^2 rays, summed in their order and scaled by 1/side^2, as lensTile
+ * does (lens.inc:364-380). rays: the planes artfHipTrace wrote; hdr: width * height each. */
+int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* const hdr[3], void* stream);
+
+/* ---- the post passes: replace bloomDownRow/bloomUpRow (src/software/software.c:320-386, 419-426)
+ * and lensFill's tone map (src/software/lens.inc:387-440), and the Vulkan shaders artfShaderBloomDown,
+ * artfShaderBloomUp and artfShaderHoleBackdrop (src/vulkan/shaders.c:2058-2142, 2207-2257). ---- */
+
+/* The bloom pyramid: frame->size[3] levels, laid out per frame->bloom[k] in three planes of
+ * frame-dependent length (artfHoleFrame's bloomPixels each). */
+int artfHipBloom(const ArtfHoleFrame* frame, float* const hdr[3], float* const bloom[3], void* stream);
+/* HDR + level 0 of the bloom to BGRA8 (alpha 255), width * height words. */
+int artfHipToneMap(const ArtfHoleFrame* frame, float* const hdr[3], float* const bloom[3], uint32_t* bgra, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
