@@ -51,6 +51,10 @@ int artfHipMemset(void* dst, int value, size_t bytes, void* stream) {
     return (int)hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
 }
 
+int artfHipCopy(void* dst, const void* src, size_t bytes, void* stream) {
+    return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream);
+}
+
 void* artfHipStreamCreate(void) {
     hipStream_t s = NULL;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return NULL;

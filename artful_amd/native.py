@@ -13,7 +13,7 @@ import numpy as np
 
 from . import abi
 
-LIB_PATH = Path(__file__).resolve().parent / "libartful_hip.so"
+LIB_PATH = Path(os.environ.get("ARTF_HIP_LIBRARY") or Path(__file__).resolve().parent / "libartful_hip.so")
 _lib = None
 
 
@@ -54,7 +54,7 @@ def lib() -> C.CDLL:
         "artfHipDeviceInfo": ([I, C.c_char_p, C.c_size_t, C.POINTER(I)], I),
         "artfHipMalloc": ([C.c_size_t], P), "artfHipFree": ([P], None),
         "artfHipUpload": ([P, P, C.c_size_t, P], I), "artfHipDownload": ([P, P, C.c_size_t, P], I),
-        "artfHipMemset": ([P, I, C.c_size_t, P], I),
+        "artfHipMemset": ([P, I, C.c_size_t, P], I), "artfHipCopy": ([P, P, C.c_size_t, P], I),
         "artfHipStreamCreate": ([], P), "artfHipStreamDestroy": ([P], None), "artfHipStreamSync": ([P], I),
         "artfHipEventCreate": ([], P), "artfHipEventDestroy": ([P], None), "artfHipEventRecord": ([P, P], I),
         "artfHipEventMs": ([P, P], F), "artfHipErrorText": ([I], C.c_char_p),

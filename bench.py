@@ -1,0 +1,201 @@
+"""Kerr black-hole tracer throughput on one MI355X, with its roofline and the reference CPU path beside it.
+
+Workload: 1920x1080, 2x2 stratified rays a pixel (8,294,400 rays a frame), spin .998, observer r=60M,
+inclination 60 deg, painted disc from the last stable orbit to 30M, Doppler 1; HDR, bloom at the
+reference's levels, tone map. A step is one frame through artfRendererDraw (trace, resolve, bloom, tone
+map) with the map and fluid already resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line. value = rays per frame x K / (wall time of the K frames), FP32 tracer.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+WIDTH, HEIGHT, SAMPLES = 1920, 1080, 2
+SPIN, RADIUS, INCLINATION, OUTER = .998, 60.0, 60.0, 30.0
+FLOPS_PER_STEP = {True: 147, False: 122}  # per Nystrom step, spin != 0 / spin == 0 (count rule in DESIGN.md)
+PEAK_FP32_TFLOPS = 157.3                    # MI355X vector FP32, published
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_scene(native, samples=SAMPLES):
+    i = math.radians(INCLINATION)
+    desc = native.hole_desc(spin=SPIN, axis=(0, 1, 0), discInner=0.0, discOuter=OUTER, doppler=1.0, samples=samples)
+    up = (0, 1, 0) if math.pi / 2 - i <= 1 else (0, 0, -1)
+    cam = native.camera((0, RADIUS * math.cos(i), RADIUS * math.sin(i)), (0, 0, 0), up, .35, .1)
+    return desc, cam
+
+
+def time_frames(native, renderer, scene, precision, steps, warmup):
+    renderer.set_tracing(precision)
+    for _ in range(warmup):
+        if not renderer.draw(scene):
+            raise RuntimeError("draw failed")
+    trace, post, slots, nsteps = [], [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        if not renderer.draw(scene):
+            raise RuntimeError("draw failed")
+        st = renderer.hole_stats()
+        trace.append(st.traceMs)
+        post.append(st.resolveMs + st.bloomMs + st.toneMs)
+        nsteps += st.steps
+        slots += st.laneSlots
+    wall = time.perf_counter() - t0
+    return {"wall_s": wall, "trace_ms": float(np.mean(trace)), "post_ms": float(np.mean(post)), "rays": int(st.rays),
+            "steps_per_frame": nsteps / steps, "active_lanes": nsteps / max(slots, 1), "bloom_levels": st.bloomLevels}
+
+
+def rel_error(a, ref, floor=1e-3):
+    rel = np.abs(a.astype(np.float64) - ref) / np.maximum(np.abs(ref), floor)
+    return {"max": float(rel.max()), "mean": float(rel.mean()), "share_over_1e-3": float((rel > 1e-3).mean())}
+
+
+def traffic_from_profiles():
+    """HBM bytes per trace launch from the committed rocprofv3 --pmc summary, or None."""
+    p = ROOT / "profiles" / "trace_1080p_traffic.md"
+    if not p.exists():
+        return None
+    m = re.search(r"bytes per launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(native, seconds=15.0):
+    """The reference's own software renderer (oracle/_ref), or the oracle where it is absent, on a
+    band of the same frame: rays/s of its trace + bloom (the reference's lensMs)."""
+    from oracle import oracle as O
+    desc, cam = workload_scene(native)
+    cores = os.cpu_count() or 1
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    if O.reference_available():
+        L = O.reference()
+        scene = O.RefScene(desc, cam)
+        # a band of the frame, at its full width and ray density: 1920 x 128, 2x2 rays a pixel
+        band_h = 128
+        r = L.refRendererNew(WIDTH, band_h, 0, threads)
+        times = []
+        t_end = time.perf_counter() + seconds
+        import ctypes as C
+        lens, raster, cpu = C.c_double(), C.c_double(), C.c_double()
+        while time.perf_counter() < t_end or len(times) < 3:
+            L.refRendererDraw(r, scene.ptr)
+            L.refRendererStats(r, C.byref(lens), C.byref(raster), C.byref(cpu))
+            times.append(lens.value)
+            if len(times) >= 30:
+                break
+        L.refRendererFree(r)
+        scene.close()
+        ms = float(np.median(times[1:] if len(times) > 2 else times))
+        rays = WIDTH * band_h * SAMPLES * SAMPLES
+        return {"value": rays / (ms / 1e3), "unit": "rays/s", "cores": threads, "kind": "reference",
+                "sample": f"reference software renderer (AVX2, {threads} threads of {cores} CPUs): {WIDTH}x{band_h} band of the "
+                          f"same view at 2x2 rays/pixel, trace+bloom (lensMs), median of {len(times) - 1} frames"}
+    scene = native.Scene(desc, cam)
+    frame, _ = scene.frame(WIDTH, HEIGHT)
+    m, (fl, _, _) = scene.map(), scene.fluid()
+    band = (HEIGHT // 2 - 32, HEIGHT // 2 + 32)
+    t0 = time.perf_counter()
+    O.trace(frame, m, fl, 32, rows=band, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = WIDTH * (band[1] - band[0]) * SAMPLES * SAMPLES
+    return {"value": rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle restatement, {threads} threads: rows {band[0]}..{band[1]} of the same frame, 2x2 rays/pixel, trace only"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the tracer on one device (multi-GPU sharding is a design "
+              "in DESIGN.md, not a measured path). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from artful_amd import native
+    if native.device_count() < 1:
+        raise SystemExit("no HIP device: bench.py needs an MI355X")
+    if args.copy_bytes:
+        L = native.lib()
+        a, b = L.artfHipMalloc(args.copy_bytes), L.artfHipMalloc(args.copy_bytes)
+        for _ in range(5):
+            if L.artfHipCopy(b, a, args.copy_bytes, None) or L.artfHipStreamSync(None):
+                raise RuntimeError("copy failed")
+        L.artfHipFree(a), L.artfHipFree(b)
+        print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 5}))
+        return
+    desc, cam = workload_scene(native)
+    scene = native.Scene(desc, cam)
+    renderer = native.Renderer(WIDTH, HEIGHT)
+    log(renderer.description())
+
+    fp32 = time_frames(native, renderer, scene, "fp32", args.steps, args.warmup)
+    rays = fp32["rays"]
+    value = rays * args.steps / fp32["wall_s"]
+    log(f"fp32: {fp32}")
+    if args.profile:
+        print(json.dumps({"rays_per_s": value, "trace_ms": fp32["trace_ms"], "steps": fp32["steps_per_frame"]}))
+        return
+    hdr32 = renderer.read_hdr()
+
+    others = {}
+    hdrs = {}
+    for p in ("fp32_fast", "mixed", "fp64"):
+        others[p] = time_frames(native, renderer, scene, p, max(3, args.steps // 4), 1)
+        hdrs[p] = renderer.read_hdr()
+        log(f"{p}: {others[p]}")
+    ref64 = hdrs["fp64"].astype(np.float64)
+    errors = {p: rel_error(h, ref64) for p, h in (("fp32", hdr32), ("fp32_fast", hdrs["fp32_fast"]), ("mixed", hdrs["mixed"]))}
+
+    flops = fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0]
+    achieved = flops / (fp32["trace_ms"] * 1e-3) / 1e12
+    traffic = traffic_from_profiles()
+    cpu = None if args.no_cpu else cpu_baseline(native)
+
+    line = {
+        "metric": "Kerr rays/s at 1080p (FP32 vs FP64 plain, <=1e-3 rel err); adjoint/forward time",
+        "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": fp32["wall_s"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "kerr-1080p-frame", "width": WIDTH, "height": HEIGHT, "rays_per_pixel": SAMPLES * SAMPLES,
+                   "spin": SPIN, "observer_r": RADIUS, "inclination_deg": INCLINATION, "disc": "painted, ISCO..30M",
+                   "doppler": 1.0, "bloom_levels": fp32["bloom_levels"], "parallelism": "single-gpu"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic},
+        "cpu_baseline": cpu,
+        "trace_rays_per_s": {"fp32": rays / (fp32["trace_ms"] * 1e-3),
+                             **{p: rays / (o["trace_ms"] * 1e-3) for p, o in others.items()}},
+        "fp32_over_fp64": others["fp64"]["trace_ms"] / fp32["trace_ms"],
+        "trace_ms": fp32["trace_ms"], "post_ms": fp32["post_ms"],
+        "steps_per_ray": fp32["steps_per_frame"] / rays, "active_lane_fraction": fp32["active_lanes"],
+        "rel_error_vs_fp64": errors,
+        "adjoint_over_forward": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,6 +27,8 @@ void artfHipFree(void* p);
 int artfHipUpload(void* dst, const void* src, size_t bytes, void* stream);
 int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream);
 int artfHipMemset(void* dst, int value, size_t bytes, void* stream);
+/* Device to device. */
+int artfHipCopy(void* dst, const void* src, size_t bytes, void* stream);
 void* artfHipStreamCreate(void);
 void artfHipStreamDestroy(void* stream);
 int artfHipStreamSync(void* stream);

@@ -102,13 +102,13 @@ static void traceRow(TraceJob* j, int y) {
                     const double nx = ((double)x + (double)dx) * (2.0 / (double)width) - 1.0;
                     const double ny = 1.0 - ((double)y + (double)dy) * (2.0 / (double)height);
                     double tr, tg, tb;
-                    ray64(f, &e64, j->law, j->map, j->fluid, nx, ny, time, &tr, &tg, &tb, &steps);
+                    ray64(f, &e64, j->law, j->map, j->fluid, nx, ny, time, &tr, &tg, &tb, &steps, NULL);
                     rd += tr, gd += tg, bd += tb;
                 } else {
                     const float nx = ((float)(x & 7) + ((float)(x & ~7) + dx)) * f->image[1] - 1.f;
                     const float ny = 1.f - ((float)y + dy) * f->image[2];
                     float tr, tg, tb;
-                    ray32(f, &e32, j->law, j->map, j->fluid, nx, ny, time, &tr, &tg, &tb, &steps);
+                    ray32(f, &e32, j->law, j->map, j->fluid, nx, ny, time, &tr, &tg, &tb, &steps, NULL);
                     r = r + tr, g = g + tg, b = b + tb;
                 }
                 if (j->steps) j->steps[(size_t)sub * width * height + (size_t)y * width + x] = (uint32_t)steps;
@@ -155,18 +155,18 @@ void oracleTrace(const ArtfHoleFrame* f, const ArtfHoleFrame64* f64, const Oracl
 }
 
 void oracleRay(const ArtfHoleFrame* f, const ArtfHoleFrame64* f64, const OracleLaw* law, int precision, const float* map, const float* fluid, double nx,
-               double ny, float time, float* rgb, int* steps) {
+               double ny, float time, float* rgb, int* steps, double* path) {
     if (precision == 64) {
         Eye64 e;
         eye64(f64, &e);
         double r, g, b;
-        ray64(f, &e, law, map, fluid, nx, ny, time, &r, &g, &b, steps);
+        ray64(f, &e, law, map, fluid, nx, ny, time, &r, &g, &b, steps, path);
         rgb[0] = (float)r, rgb[1] = (float)g, rgb[2] = (float)b;
     } else {
         Eye32 e;
         eye32(f, &e);
         float r, g, b;
-        ray32(f, &e, law, map, fluid, (float)nx, (float)ny, time, &r, &g, &b, steps);
+        ray32(f, &e, law, map, fluid, (float)nx, (float)ny, time, &r, &g, &b, steps, path);
         rgb[0] = r, rgb[1] = g, rgb[2] = b;
     }
 }

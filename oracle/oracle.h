@@ -29,9 +29,11 @@ typedef struct OracleLaw {
 void oracleTrace(const ArtfHoleFrame* frame, const ArtfHoleFrame64* frame64, const OracleLaw* law, int precision, const float* map, const float* fluid,
                  float* red, float* green, float* blue, uint32_t* steps, int rowBegin, int rowEnd, int threads);
 
-/* One ray, for tests: its light and its steps. */
+/* One ray, for tests: its light, its steps, and (path: NULL or 6 doubles) where it left on the unit
+ * sphere, its drag angle there, 1 if it left (0: captured, stopped or out of steps), and what of the
+ * light behind got through. */
 void oracleRay(const ArtfHoleFrame* frame, const ArtfHoleFrame64* frame64, const OracleLaw* law, int precision, const float* map, const float* fluid,
-               double nx, double ny, float time, float* rgb, int* steps);
+               double nx, double ny, float time, float* rgb, int* steps, double* path);
 
 /* The bloom pyramid into three planes of the frame's bloomPixels each. */
 void oracleBloom(const ArtfHoleFrame* frame, const float* const hdr[3], float* const bloom[3]);

@@ -35,7 +35,7 @@ def oracle() -> C.CDLL:
                                   C.c_int, C.c_int, C.c_int]
         L.oracleTrace.restype = None
         L.oracleRay.argtypes = [C.POINTER(abi.HoleFrame), C.POINTER(abi.HoleFrame64), C.POINTER(Law), C.c_int, P, P, C.c_double,
-                                C.c_double, C.c_float, P, C.POINTER(C.c_int)]
+                                C.c_double, C.c_float, P, C.POINTER(C.c_int), P]
         L.oracleRay.restype = None
         L.oracleBloom.argtypes = [C.POINTER(abi.HoleFrame), P * 3, P * 3]
         L.oracleBloom.restype = None
@@ -66,12 +66,15 @@ def trace(frame: abi.HoleFrame, map_: np.ndarray, fluid: np.ndarray, precision: 
 
 
 def ray(frame, map_, fluid, nx, ny, time, precision=32, frame64=None, lw=None):
+    """(rgb, steps, path): path = where the ray left on the unit sphere (3), its drag angle, 1.0 if it
+    left, and the transmittance."""
     rgb = np.zeros(3, np.float32)
+    path = np.zeros(6, np.float64)
     n = C.c_int()
     oracle().oracleRay(C.byref(frame), C.byref(frame64) if frame64 is not None else None, C.byref(lw or law()), precision,
                        np.ascontiguousarray(map_, np.float32).ctypes.data, np.ascontiguousarray(fluid, np.float32).ctypes.data,
-                       nx, ny, time, rgb.ctypes.data, C.byref(n))
-    return rgb, n.value
+                       nx, ny, time, rgb.ctypes.data, C.byref(n), path.ctypes.data)
+    return rgb, n.value, path
 
 
 def bloom(frame: abi.HoleFrame, hdr: np.ndarray, bloom_pixels: int) -> np.ndarray:
