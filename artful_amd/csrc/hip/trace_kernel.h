@@ -24,8 +24,8 @@ __device__ __forceinline__ void subPixel(uint32_t seed, int px, int py, int sx, 
     dy = ((float)sy + jy) / (float)side;
 }
 
-/* The wave's step count and its lane-slots (its longest ray times its lanes with a pixel), added to
- * two counters by one lane. */
+/* The wave's step count and its lane-slots (its longest ray times its lanes with a pixel), added by
+ * one lane to the counter slot of its workgroup (slots spread the atomics over many addresses). */
 __device__ __forceinline__ void countSteps(unsigned long long* counters, int steps, bool valid) {
     if (!counters) return;
     unsigned long long sum = (unsigned long long)steps;
@@ -36,8 +36,9 @@ __device__ __forceinline__ void countSteps(unsigned long long* counters, int ste
         lanes += __shfl_xor(lanes, offset);
     }
     if ((threadIdx.x & 63) == 0 && lanes) {
-        atomicAdd(&counters[0], sum);
-        atomicAdd(&counters[1], (unsigned long long)most * (unsigned long long)lanes);
+        const unsigned slot = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x + (threadIdx.x >> 6)) % ARTF_HIP_COUNTER_SLOTS;
+        atomicAdd(&counters[slot], sum);
+        atomicAdd(&counters[ARTF_HIP_COUNTER_SLOTS + slot], (unsigned long long)most * (unsigned long long)lanes);
     }
 }
 

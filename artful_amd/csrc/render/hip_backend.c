@@ -42,7 +42,8 @@ typedef struct HipRenderer {
     size_t bloomPixels;
     uint32_t* bgra;
     size_t bgraPixels;
-    unsigned long long* counters;
+    unsigned long long* counters;     /* device: 2 * ARTF_HIP_COUNTER_SLOTS */
+    unsigned long long* countersHost; /* their copy, summed on the host */
     /* the last frame */
     bool drawn, hole;
     uint32_t background;
@@ -137,7 +138,7 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
         h->bloomPixels = bloomPixels;
     }
     if (!h->counters) {
-        h->counters = artfHipMalloc(2 * sizeof(unsigned long long));
+        h->counters = artfHipMalloc(2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long));
         if (!h->counters) return ok(h, 2, "allocate");
     }
     return true;
@@ -203,7 +204,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     a.emissivity = h->tracing.emissivity;
     a.jitter = h->tracing.jitter;
 
-    if (!ok(h, artfHipMemset(h->counters, 0, 2 * sizeof(unsigned long long), h->stream), "clear")) return false;
+    if (!ok(h, artfHipMemset(h->counters, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long), h->stream), "clear")) return false;
     if (!ok(h, artfHipEventRecord(h->events[EV_START], h->stream), "event")) return false;
     if (!ok(h, artfHipTrace(&a, h->stream), "trace")) return false;
     if (!ok(h, artfHipEventRecord(h->events[EV_TRACED], h->stream), "event")) return false;
@@ -213,8 +214,10 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     if (!ok(h, artfHipEventRecord(h->events[EV_BLOOMED], h->stream), "event")) return false;
     if (!ok(h, artfHipToneMap(&h->frame, h->hdr, h->bloom, h->bgra, h->stream), "tone map")) return false;
     if (!ok(h, artfHipEventRecord(h->events[EV_TONED], h->stream), "event")) return false;
+    unsigned long long* slots = h->countersHost;
+    if (!ok(h, artfHipDownload(slots, h->counters, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *slots, h->stream), "wait")) return false;
     unsigned long long counts[2] = {0, 0};
-    if (!ok(h, artfHipDownload(counts, h->counters, sizeof counts, h->stream), "wait")) return false;
+    for (int i = 0; i < ARTF_HIP_COUNTER_SLOTS; ++i) counts[0] += slots[i], counts[1] += slots[ARTF_HIP_COUNTER_SLOTS + i];
 
     ArtfHoleStats* st = &h->holeStats;
     st->traceMs = artfHipEventMs(h->events[EV_START], h->events[EV_TRACED]);
@@ -316,6 +319,7 @@ static void hipDestroy(ArtfRenderer* r) {
     for (int i = 0; i < EV_COUNT; ++i) artfHipEventDestroy(h->events[i]);
     artfHipStreamDestroy(h->stream);
     artfFree(h->overlay);
+    artfFree(h->countersHost);
     artfFree(h);
 }
 
@@ -350,6 +354,7 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     artfHipDeviceInfo(o->device, name, sizeof name, &units);
     snprintf(h->description, sizeof h->description, "HIP, %s, %d compute units", name, units);
     h->tracing = artfHoleTracingDefaults();
+    h->countersHost = artfCalloc(2 * ARTF_HIP_COUNTER_SLOTS, sizeof *h->countersHost);
     hipResize(&h->base, o->width, o->height);
     return &h->base;
 }

@@ -45,6 +45,8 @@ const char* artfHipErrorText(int status);
  * shader artfShaderHoleTrace (src/vulkan/shaders.c:1957-2026) dispatched by traceHole
  * (src/vulkan/vulkan.c:1758-1783). ---- */
 
+#define ARTF_HIP_COUNTER_SLOTS 1024
+
 typedef struct ArtfHipTraceArgs {
     ArtfHoleFrame frame;     /* from artfHoleFrame: FP32 uniforms */
     ArtfHoleFrame64 frame64; /* from artfHoleFrame64: read by the MIXED and FP64 tracers */
@@ -54,8 +56,10 @@ typedef struct ArtfHipTraceArgs {
      * a pixel: each ray's light, side^2 * width * height each, ray k of pixel p at k * width * height + p
      * (artfHipResolve then averages them in the reference's order). */
     float* out[3];
-    /* device, 2 words, added to: the rays' steps, and the lane-slots their waves spent stepping (so
-     * steps / slots is the stepping loop's active-lane fraction). NULL: not counted. */
+    /* device, 2 * ARTF_HIP_COUNTER_SLOTS words, added to: the rays' steps in the first half, and the
+     * lane-slots their waves spent stepping in the second (steps / slots is the stepping loop's
+     * active-lane fraction); the totals are the sums of each half. Spread over slots so that waves do
+     * not queue on one address. NULL: not counted. */
     unsigned long long* counters;
     int precision;      /* ArtfPrecision (artful/renderer.h) */
     int maxSteps;       /* 0: 192 */
