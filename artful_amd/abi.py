@@ -78,11 +78,20 @@ class Overlay(C.Structure):
                 ("partial", C.c_bool), ("damage", C.c_void_p), ("damageCount", C.c_size_t)]
 
 
+class HoleGradient(C.Structure):
+    _fields_ = [("spin", C.c_double), ("position", C.c_double * 3), ("discInner", C.c_double), ("discOuter", C.c_double),
+                ("emissivity", C.c_double), ("gain", C.c_double), ("frame", C.c_double * 26), ("spinDirect", C.c_double)]
+
+
 class HipTraceArgs(C.Structure):
     _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("map", C.c_void_p), ("fluid", C.c_void_p),
                 ("out", C.c_void_p * 3), ("counters", C.c_void_p), ("precision", C.c_int), ("maxSteps", C.c_int),
                 ("escapeRadius", C.c_float), ("disc", C.c_int), ("emissivity", C.c_float), ("jitter", C.c_uint32),
                 ("rowBegin", C.c_int), ("rowEnd", C.c_int)]
+
+
+class HipAdjointArgs(C.Structure):
+    _fields_ = [("trace", HipTraceArgs), ("seed", C.c_void_p * 3), ("partials", C.c_void_p)]
 
 
 # enum values (include/artful/renderer.h, include/artful/hole.h)

@@ -99,6 +99,20 @@ int artfHipTrace(const ArtfHipTraceArgs* a, void* stream) {
     return (int)launchTraceFast(*a, (hipStream_t)stream);
 }
 
+int artfHipAdjointBlocks(const ArtfHoleFrame* f) { return frameFits(f) ? adjointBlocks(*f) : 0; }
+
+int artfHipAdjoint(const ArtfHipAdjointArgs* args, void* stream) {
+    if (!args) return (int)hipErrorInvalidValue;
+    const ArtfHipTraceArgs* a = &args->trace;
+    if (!frameFits(&a->frame) || !a->map || !a->out[0] || !a->out[1] || !a->out[2] || !args->seed[0] || !args->seed[1] || !args->seed[2] ||
+        !args->partials)
+        return (int)hipErrorInvalidValue;
+    if (a->disc != ARTF_DISC_ANALYTIC || (a->precision != ARTF_PRECISION_FP32 && a->precision != ARTF_PRECISION_FP64)) return (int)hipErrorInvalidValue;
+    if (a->maxSteps < 0 || a->maxSteps > 2048 || a->frame.look[2] > 0.f || a->frame.boost[3] > 1.f || a->rowBegin != 0 || a->rowEnd != 0)
+        return (int)hipErrorInvalidValue;
+    return (int)launchAdjoint(*args, (hipStream_t)stream);
+}
+
 int artfHipResolve(const ArtfHoleFrame* f, float* const rays[3], float* const hdr[3], void* stream) {
     if (!frameFits(f) || !rays || !hdr) return (int)hipErrorInvalidValue;
     return (int)launchResolve(*f, rays, hdr, (hipStream_t)stream);

@@ -1,0 +1,94 @@
+/* artful-hip — the adjoint kernel: each lane runs one ray forward and back (adjoint.h), and each
+ * workgroup sums its rays' gradients into 32 doubles (a fixed order: the sums are the same every run).
+ * Built without contraction, so its forward light equals trace_exact.hip's analytic-disc light. */
+#include "adjoint.h"
+#include "trace_kernel.h"
+
+namespace artf {
+
+template <typename R>
+__global__ __launch_bounds__(kGroup) void adjointKernel(ArtfHipAdjointArgs args) {
+    const ArtfHipTraceArgs& a = args.trace;
+    const ArtfHoleFrame& f = a.frame;
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+    const int sub = blockIdx.z, sx = sub % side, sy = sub / side;
+    const bool valid = px < width && py < height;
+
+    TraceLaw law;
+    law.maxSteps = a.maxSteps > 0 ? a.maxSteps : 192;
+    law.uEscape = a.escapeRadius > 0.f ? 1.f / a.escapeRadius : 0.f;
+    law.disc = DISC_ANALYTIC;
+    law.q = a.emissivity;
+
+    double grad[ADJ_COUNT];
+    for (int k = 0; k < ADJ_COUNT; ++k) grad[k] = 0.0;
+    if (valid) {
+        float dx, dy;
+        subPixel(a.jitter, px, py, sx, sy, side, dx, dy);
+        const float moment = ((float)((sub * 5) % (side * side)) + .5f) * f.image[3] - .5f;
+        const float time = f.disc[2] + f.spin[3] * moment;
+        const size_t pixel = (size_t)py * width + px;
+        float light[3];
+        if (sizeof(R) == 8) {
+            Eye<double> e;
+            eyeFrom64(a.frame64, e);
+            const double nx = ((double)px + (double)dx) * (2.0 / (double)width) - 1.0;
+            const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
+            const double seed[3] = {(double)args.seed[0][pixel] * f.image[3], (double)args.seed[1][pixel] * f.image[3],
+                                    (double)args.seed[2][pixel] * f.image[3]};
+            Saved<double> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT];
+            double g[ADJ_COUNT];
+            for (int k = 0; k < ADJ_COUNT; ++k) g[k] = 0.0;
+            adjointRay<double>(f, e, law, a.map, nx, ny, time, seed, light, g, ckpt, seg);
+            for (int k = 0; k < ADJ_COUNT; ++k) grad[k] = g[k];
+        } else {
+            Eye<float> e;
+            eyeFrom<float>(f, e);
+            const float nx = ((float)(px & 7) + ((float)(px & ~7) + dx)) * f.image[1] - 1.f;
+            const float ny = 1.f - ((float)py + dy) * f.image[2];
+            const float seed[3] = {args.seed[0][pixel] * f.image[3], args.seed[1][pixel] * f.image[3], args.seed[2][pixel] * f.image[3]};
+            Saved<float> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT];
+            float g[ADJ_COUNT];
+            for (int k = 0; k < ADJ_COUNT; ++k) g[k] = 0.f;
+            adjointRay<float>(f, e, law, a.map, nx, ny, time, seed, light, g, ckpt, seg);
+            for (int k = 0; k < ADJ_COUNT; ++k) grad[k] = (double)g[k];
+        }
+        const size_t at = (side == 1 ? 0 : (size_t)sub * width * height) + pixel;
+        a.out[0][at] = light[0];
+        a.out[1][at] = light[1];
+        a.out[2][at] = light[2];
+    }
+    /* the workgroup's sum, in a fixed order: across each wave, then the four waves */
+    __shared__ double waves[4][32];
+    for (int k = 0; k < ADJ_COUNT; ++k) {
+        double v = grad[k];
+        for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset);
+        if (lane == 0) waves[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int k = threadIdx.x;
+        const double sum = k < ADJ_COUNT ? ((waves[0][k] + waves[1][k]) + (waves[2][k] + waves[3][k])) : 0.0;
+        const size_t block = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        args.partials[block * 32 + k] = sum;
+    }
+}
+
+int adjointBlocks(const ArtfHoleFrame& f) {
+    const int side = (int)f.size[2];
+    return (int)(((f.size[0] + kTile - 1) / kTile) * ((f.size[1] + kTile - 1) / kTile)) * side * side;
+}
+
+hipError_t launchAdjoint(const ArtfHipAdjointArgs& args, hipStream_t stream) {
+    const ArtfHoleFrame& f = args.trace.frame;
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+    const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((height + kTile - 1) / kTile), (unsigned)(side * side));
+    if (args.trace.precision == ARTF_PRECISION_FP64) hipLaunchKernelGGL(adjointKernel<double>, grid, dim3(kGroup), 0, stream, args);
+    else hipLaunchKernelGGL(adjointKernel<float>, grid, dim3(kGroup), 0, stream, args);
+    return hipGetLastError();
+}
+
+} // namespace artf

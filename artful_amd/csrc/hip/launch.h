@@ -16,6 +16,9 @@ constexpr int kGroup = 256;
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
 /* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */
 hipError_t launchTraceFast(const ArtfHipTraceArgs& a, hipStream_t stream);
+/* The analytic-disc adjoint (adjoint.hip, contraction off). */
+hipError_t launchAdjoint(const ArtfHipAdjointArgs& a, hipStream_t stream);
+int adjointBlocks(const ArtfHoleFrame& f);
 /* Resolve, bloom and tone map (post.hip, contraction off). */
 hipError_t launchResolve(const ArtfHoleFrame& f, float* const rays[3], float* const hdr[3], hipStream_t stream);
 hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], hipStream_t stream);

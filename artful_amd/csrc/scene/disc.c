@@ -127,6 +127,14 @@ static void paintRow(void* user, uint32_t y, uint32_t worker) {
     }
 }
 
+/* Colours by heat, linear in kelvin from 0 (the kelvin goes through a float, as artfMaxf's argument). */
+void artfDiscColors(float* colors) {
+    for (int i = 0; i < ARTF_HOLE_COLORS; ++i) {
+        const double kelvin = artfMaxf((float)(ARTF_HOLE_KELVIN * (double)i / (ARTF_HOLE_COLORS / 2)), 400.f);
+        blackBodyRgb(kelvin, colors + i * 3);
+    }
+}
+
 void artfDiscPaint(float* map) {
     ArtfPool* pool = artfPoolNew(0);
     artfPoolRun(pool, paintRow, map, ARTF_HOLE_MAP_HEIGHT);
@@ -149,12 +157,7 @@ void artfDiscPaint(float* map) {
         }
     }
 
-    /* colours by heat, linear in kelvin from 0 (the kelvin goes through a float, as artfMaxf's argument) */
-    float* colors = map + ARTF_HOLE_COLORS_AT;
-    for (int i = 0; i < ARTF_HOLE_COLORS; ++i) {
-        const double kelvin = artfMaxf((float)(ARTF_HOLE_KELVIN * (double)i / (ARTF_HOLE_COLORS / 2)), 400.f);
-        blackBodyRgb(kelvin, colors + i * 3);
-    }
+    artfDiscColors(map + ARTF_HOLE_COLORS_AT);
 
     /* the sky: faint dust heaped along a wandering band */
     float* sky = map + ARTF_HOLE_SKY_AT;

@@ -50,26 +50,32 @@ ArtfBlackHoleDesc artfBlackHoleDefaults(void) {
     return d;
 }
 
+/* The description as a scene keeps it (hole.c:245-258): spin within +-.998, the inner edge the last
+ * stable orbit or at least .2 past the horizon, the outer at least a mass further, samples 1..8. */
+ArtfBlackHoleDesc artfHoleClamped(const ArtfBlackHoleDesc* desc) {
+    ArtfBlackHoleDesc d = *desc;
+    d.spin = artfClampf(desc->spin, -.998f, .998f);
+    const float horizon = 1.f + sqrtf(1.f - d.spin * d.spin);
+    d.discInner = desc->discInner > 0 ? artfMaxf(desc->discInner, horizon + .2f) : artfHoleIsco(d.spin);
+    d.discOuter = artfMaxf(desc->discOuter, d.discInner + 1.f);
+    d.mass = artfMaxf(desc->mass, 1e-6f);
+    d.discHeight = artfMaxf(desc->discHeight, 1e-3f);
+    d.samples = artfClampi(desc->samples, 1, 8);
+    return d;
+}
+
 void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     ArtfHole* h = &s->hole;
     if (!desc) {
         h->on = false;
         return;
     }
-    const float spin = artfClampf(desc->spin, -.998f, .998f);
-    const float horizon = 1.f + sqrtf(1.f - spin * spin);
-    const float inner = desc->discInner > 0 ? artfMaxf(desc->discInner, horizon + .2f) : artfHoleIsco(spin);
-    const float outer = artfMaxf(desc->discOuter, inner + 1.f);
+    const ArtfBlackHoleDesc clamped = artfHoleClamped(desc);
+    const float spin = clamped.spin, inner = clamped.discInner, outer = clamped.discOuter;
     const bool sameDisc = h->map && h->desc.discInner == inner && h->desc.discOuter == outer;
     const bool sameSpin = sameDisc && h->desc.spin == spin;
     h->on = true;
-    h->desc = *desc;
-    h->desc.spin = spin;
-    h->desc.mass = artfMaxf(desc->mass, 1e-6f);
-    h->desc.discInner = inner;
-    h->desc.discOuter = outer;
-    h->desc.discHeight = artfMaxf(desc->discHeight, 1e-3f);
-    h->desc.samples = artfClampi(desc->samples, 1, 8);
+    h->desc = clamped;
     if (!sameSpin) {
         if (!h->map) {
             h->map = artfMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
@@ -87,10 +93,8 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
 
 /* The eye's local frame and the Kerr metric there, in either precision. HOLE_REAL is float or double;
  * the float form is the reference's arithmetic order exactly. */
-#define HOLE_FRAME_BODY(HOLE_REAL, SQRT, TAN, FABS)                                                           \
-    const ArtfBlackHoleDesc* d = &s->hole.desc;                                                              \
-    const ArtfCamera* c = &s->camera;                                                                        \
-    const HOLE_REAL a = d->spin, horizon = (HOLE_REAL)1 + SQRT((HOLE_REAL)1 - a * a);                        \
+#define HOLE_FRAME_BODY(HOLE_REAL, SQRT, TAN, FABS, SPIN, EYE)                                               \
+    const HOLE_REAL a = SPIN, horizon = (HOLE_REAL)1 + SQRT((HOLE_REAL)1 - a * a);                        \
     /* the hole's own axes: the disc in x-z, its axis y */                                                   \
     HOLE_REAL gx = d->axis.x, gy = d->axis.y, gz = d->axis.z;                                                \
     {                                                                                                        \
@@ -114,9 +118,9 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     const HOLE_REAL eyx = gx, eyy = gy, eyz = gz;                                                            \
     const HOLE_REAL ezx = exy * eyz - exz * eyy, ezy = exz * eyx - exx * eyz, ezz = exx * eyy - exy * eyx;    \
     const HOLE_REAL invMass = (HOLE_REAL)1 / (HOLE_REAL)d->mass;                                             \
-    HOLE_REAL vx = ((HOLE_REAL)c->position.x - (HOLE_REAL)d->position.x) * invMass;                          \
-    HOLE_REAL vy = ((HOLE_REAL)c->position.y - (HOLE_REAL)d->position.y) * invMass;                          \
-    HOLE_REAL vz = ((HOLE_REAL)c->position.z - (HOLE_REAL)d->position.z) * invMass;                          \
+    HOLE_REAL vx = ((HOLE_REAL)EYE(x) - (HOLE_REAL)d->position.x) * invMass;                                 \
+    HOLE_REAL vy = ((HOLE_REAL)EYE(y) - (HOLE_REAL)d->position.y) * invMass;                                 \
+    HOLE_REAL vz = ((HOLE_REAL)EYE(z) - (HOLE_REAL)d->position.z) * invMass;                                 \
     HOLE_REAL r = SQRT(vx * vx + vy * vy + vz * vz);                                                         \
     const HOLE_REAL least = horizon + (HOLE_REAL).3f;                                                        \
     if (r < least) { /* the eye stays just outside the horizon */                                           \
@@ -151,9 +155,9 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     out->spin[0] = a, out->spin[1] = (HOLE_REAL).995f / horizon;                                             \
     out->spin[2] = d->discHeight, out->spin[3] = d->shutter;                                                 \
     /* camera axes, scaled by the half-angles, then taken into (r, theta, phi) at the eye */               \
-    HOLE_REAL fx = (HOLE_REAL)c->target.x - (HOLE_REAL)c->position.x;                                        \
-    HOLE_REAL fy = (HOLE_REAL)c->target.y - (HOLE_REAL)c->position.y;                                        \
-    HOLE_REAL fz = (HOLE_REAL)c->target.z - (HOLE_REAL)c->position.z;                                        \
+    HOLE_REAL fx = (HOLE_REAL)c->target.x - (HOLE_REAL)EYE(x);                                               \
+    HOLE_REAL fy = (HOLE_REAL)c->target.y - (HOLE_REAL)EYE(y);                                               \
+    HOLE_REAL fz = (HOLE_REAL)c->target.z - (HOLE_REAL)EYE(z);                                               \
     {                                                                                                        \
         const HOLE_REAL l = SQRT(fx * fx + fy * fy + fz * fz);                                               \
         if (l > 0) {                                                                                         \
@@ -195,11 +199,12 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     out->boost[3] = (HOLE_REAL)1 / SQRT((HOLE_REAL)1 - speed * speed);                                       \
     out->disc[0] = d->discInner, out->disc[1] = d->discOuter, out->disc[2] = d->time, out->disc[3] = d->discThickness;
 
-void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCap, ArtfHoleFrame* out, uint32_t* bloomPixels) {
+void artfHoleFrameOf(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double fluidTime, int width, int height, int levelsCap, ArtfHoleFrame* out,
+                     uint32_t* bloomPixels) {
     memset(out, 0, sizeof *out);
-    HOLE_FRAME_BODY(float, sqrtf, tanf, fabsf)
-    const ArtfFluid* fluid = s->hole.fluid;
-    out->flow[0] = fluid ? (float)artfFluidTime(fluid) : 0.f;
+#define CAMERA_EYE(k) c->position.k
+    HOLE_FRAME_BODY(float, sqrtf, tanf, fabsf, d->spin, CAMERA_EYE)
+    out->flow[0] = (float)fluidTime;
     out->look[0] = d->discGain, out->look[1] = d->doppler, out->look[2] = d->stars, out->look[3] = d->bloom;
     out->image[0] = d->exposure;
     out->image[1] = 2.f / (float)width;
@@ -218,14 +223,29 @@ void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCa
         ++levels;
     }
     out->size[3] = levels;
-    *bloomPixels = at;
+    if (bloomPixels) *bloomPixels = at;
+}
+
+void artfHoleFrame64Of(const ArtfBlackHoleDesc* d, const ArtfCamera* c, int width, int height, ArtfHoleFrame64* out) {
+    memset(out, 0, sizeof *out);
+    HOLE_FRAME_BODY(double, sqrt, tan, fabs, (double)d->spin, CAMERA_EYE)
+}
+
+/* The FP64 frame as a function of a spin and an eye position given in doubles (for derivatives). */
+void artfHoleFrame64At(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double spin, const double eye[3], int width, int height, ArtfHoleFrame64* out) {
+    memset(out, 0, sizeof *out);
+#define GIVEN_EYE(k) eye[#k[0] - 'x']
+    HOLE_FRAME_BODY(double, sqrt, tan, fabs, spin, GIVEN_EYE)
+#undef GIVEN_EYE
+}
+
+void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCap, ArtfHoleFrame* out, uint32_t* bloomPixels) {
+    const double t = s->hole.fluid ? artfFluidTime(s->hole.fluid) : 0.0;
+    artfHoleFrameOf(&s->hole.desc, &s->camera, t, width, height, levelsCap, out, bloomPixels);
 }
 
 void artfHoleFrame(const ArtfScene* s, int width, int height, ArtfHoleFrame* out, uint32_t* bloomPixels) {
     artfHoleFrameLevels(s, width, height, 0, out, bloomPixels);
 }
 
-void artfHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out) {
-    memset(out, 0, sizeof *out);
-    HOLE_FRAME_BODY(double, sqrt, tan, fabs)
-}
+void artfHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out) { artfHoleFrame64Of(&s->hole.desc, &s->camera, width, height, out); }

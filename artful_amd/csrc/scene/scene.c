@@ -50,3 +50,9 @@ uint32_t artfSceneHoleFrame(const ArtfScene* s, int width, int height, int level
 }
 
 void artfSceneHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out) { artfHoleFrame64(s, width, height, out); }
+
+void artfHoleFrameFor(const ArtfBlackHoleDesc* desc, const ArtfCamera* camera, int width, int height, ArtfHoleFrame* out, ArtfHoleFrame64* out64) {
+    const ArtfBlackHoleDesc d = artfHoleClamped(desc);
+    if (out) artfHoleFrameOf(&d, camera, 0.0, width, height, 0, out, NULL);
+    if (out64) artfHoleFrame64Of(&d, camera, width, height, out64);
+}

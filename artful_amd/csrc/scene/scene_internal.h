@@ -43,5 +43,13 @@ void artfHoleFrame(const ArtfScene* s, int width, int height, ArtfHoleFrame* out
 void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCap, ArtfHoleFrame* out, uint32_t* bloomPixels);
 /* The same geometry from the same description, worked out in doubles. */
 void artfHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out);
+/* The same, from a description (already clamped: artfHoleClamped) and a camera rather than a scene. */
+ArtfBlackHoleDesc artfHoleClamped(const ArtfBlackHoleDesc* desc);
+void artfHoleFrameOf(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double fluidTime, int width, int height, int levelsCap, ArtfHoleFrame* out,
+                     uint32_t* bloomPixels);
+void artfHoleFrame64Of(const ArtfBlackHoleDesc* d, const ArtfCamera* c, int width, int height, ArtfHoleFrame64* out);
+void artfHoleFrame64At(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double spin, const double eye[3], int width, int height, ArtfHoleFrame64* out);
+/* The black-body colour table alone (ARTF_HOLE_COLORS x RGB), as the map holds it. */
+void artfDiscColors(float* colors);
 
 #endif

@@ -37,6 +37,8 @@ def lib() -> C.CDLL:
         "artfSceneHoleFluid": ([P, C.POINTER(U64), C.POINTER(D)], C.POINTER(C.c_float)),
         "artfSceneHoleFrame": ([P, I, I, I, C.POINTER(abi.HoleFrame)], U32),
         "artfSceneHoleFrame64": ([P, I, I, C.POINTER(abi.HoleFrame64)], None),
+        "artfHoleFrameFor": ([C.POINTER(abi.BlackHoleDesc), C.POINTER(abi.Camera), I, I, C.POINTER(abi.HoleFrame),
+                              C.POINTER(abi.HoleFrame64)], None),
         "artfFluidNew": ([F, F, F, D], P), "artfFluidFree": ([P], None), "artfFluidAdvance": ([P, D], B),
         "artfFluidField": ([P], C.POINTER(C.c_float)), "artfFluidTime": ([P], D),
         "artfRendererDefaults": ([I, I], abi.RendererOptions),
@@ -62,6 +64,13 @@ def lib() -> C.CDLL:
         "artfHipResolve": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),
         "artfHipBloom": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),
         "artfHipToneMap": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P, P], I),
+        "artfHipAdjointBlocks": ([C.POINTER(abi.HoleFrame)], I),
+        "artfHipAdjoint": ([C.POINTER(abi.HipAdjointArgs), P], I),
+        "artfHoleAdjointNew": ([I, I, I], P), "artfHoleAdjointFree": ([P], None),
+        "artfHoleAdjointRender": ([P, C.POINTER(abi.BlackHoleDesc), C.POINTER(abi.Camera), C.POINTER(abi.HoleTracing), P], B),
+        "artfHoleAdjointGradient": ([P, C.POINTER(abi.BlackHoleDesc), C.POINTER(abi.Camera), C.POINTER(abi.HoleTracing), P, P,
+                                     C.POINTER(abi.HoleGradient)], B),
+        "artfHoleAdjointTimes": ([P, C.POINTER(D), C.POINTER(D)], None),
     }
     for name, (args, ret) in sig.items():
         fn = getattr(L, name)
@@ -163,6 +172,26 @@ class Scene:
             pass
 
 
+def frames_for(desc, cam, width, height):
+    """(ArtfHoleFrame, ArtfHoleFrame64) for a description and camera, without a scene."""
+    f, f64 = abi.HoleFrame(), abi.HoleFrame64()
+    lib().artfHoleFrameFor(C.byref(desc), C.byref(cam), width, height, C.byref(f), C.byref(f64))
+    return f, f64
+
+
+def black_body_colors() -> np.ndarray:
+    """The map's colour table (what the analytic disc reads), from a scene built once."""
+    global _colors
+    if _colors is None:
+        s = Scene(hole_desc(), camera((0, 0, 30)))
+        _colors = s.map()[abi.COLORS_AT:abi.SKY_AT].copy()
+        s.close()
+    return _colors
+
+
+_colors = None
+
+
 def device_count() -> int:
     return lib().artfHipDeviceCount()
 
@@ -232,6 +261,56 @@ class Renderer:
     def close(self) -> None:
         if self.ptr:
             self._L.artfRendererFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tracing(precision="fp32", disc="analytic", emissivity=3.0, max_steps=0, escape_radius=0.0, jitter=0, bloom_levels=0):
+    t = lib().artfHoleTracingDefaults()
+    t.precision = abi.PRECISION[precision] if isinstance(precision, str) else precision
+    t.disc = abi.DISC[disc] if isinstance(disc, str) else disc
+    t.emissivity, t.maxSteps, t.escapeRadius, t.jitter, t.bloomLevels = emissivity, max_steps, escape_radius, jitter, bloom_levels
+    return t
+
+
+class HoleAdjoint:
+    """The differentiable analytic-disc renderer (include/artful/adjoint.h)."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        self._L = lib()
+        self.ptr = self._L.artfHoleAdjointNew(width, height, device)
+        if not self.ptr:
+            raise RuntimeError("artfHoleAdjointNew: no HIP device")
+        self.width, self.height = width, height
+
+    def render(self, desc, cam, trc) -> np.ndarray:
+        out = np.zeros((3, self.height, self.width), np.float32)
+        if not self._L.artfHoleAdjointRender(self.ptr, C.byref(desc), C.byref(cam), C.byref(trc), out.ctypes.data):
+            raise RuntimeError("artfHoleAdjointRender failed")
+        return out
+
+    def gradient(self, desc, cam, trc, seed: np.ndarray):
+        seed = np.ascontiguousarray(seed, np.float32)
+        assert seed.shape == (3, self.height, self.width)
+        hdr = np.zeros((3, self.height, self.width), np.float32)
+        g = abi.HoleGradient()
+        if not self._L.artfHoleAdjointGradient(self.ptr, C.byref(desc), C.byref(cam), C.byref(trc), seed.ctypes.data, hdr.ctypes.data, C.byref(g)):
+            raise RuntimeError("artfHoleAdjointGradient failed")
+        return g, hdr
+
+    def times(self) -> tuple[float, float]:
+        f, a = C.c_double(), C.c_double()
+        self._L.artfHoleAdjointTimes(self.ptr, C.byref(f), C.byref(a))
+        return f.value, a.value
+
+    def close(self) -> None:
+        if self.ptr:
+            self._L.artfHoleAdjointFree(self.ptr)
             self.ptr = None
 
     def __del__(self):

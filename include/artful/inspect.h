@@ -22,6 +22,9 @@ const float* artfSceneHoleFluid(const ArtfScene* s, uint64_t* version, double* t
 uint32_t artfSceneHoleFrame(const ArtfScene* s, int width, int height, int levels, ArtfHoleFrame* out);
 /* The same geometry in doubles. */
 void artfSceneHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64* out);
+/* The frame's uniforms, FP32 and FP64, straight from a description (clamped as a scene would) and a
+ * camera, without a scene (no map or fluid is built; the fluid's moment is taken as 0). */
+void artfHoleFrameFor(const ArtfBlackHoleDesc* desc, const ArtfCamera* camera, int width, int height, ArtfHoleFrame* out, ArtfHoleFrame64* out64);
 /* Artful's last stable orbit (Bardeen-Press-Teukolsky). */
 float artfHoleIsco(float spin);
 

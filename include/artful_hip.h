@@ -76,6 +76,23 @@ int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
  * does (lens.inc:364-380). rays: the planes artfHipTrace wrote; hdr: width * height each. */
 int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* const hdr[3], void* stream);
 
+/* ---- the adjoint (not in the reference): image-loss gradients of the analytic-disc tracer ---- */
+
+typedef struct ArtfHipAdjointArgs {
+    /* the ray law and buffers, as for artfHipTrace: disc must be ARTF_DISC_ANALYTIC, precision FP32 or
+     * FP64, the frame's stars 0 and its eye at rest (boost w 1), maxSteps at most 2048. out[] receives
+     * the rays' light, as artfHipTrace writes it. */
+    ArtfHipTraceArgs trace;
+    const float* seed[3]; /* device: dL/d(HDR) of each pixel, R G B planes of width*height */
+    /* device: 32 doubles per workgroup (artfHipAdjointBlocks of them), each workgroup's sum over its
+     * rays of dL/d: forward[3], right[3], up[3], metric[4], place[4], er[3], etheta[3], ephi[3] (the
+     * frame's fields), the spin where the rays read it directly, disc inner, outer, q, gain. */
+    double* partials;
+} ArtfHipAdjointArgs;
+
+int artfHipAdjointBlocks(const ArtfHoleFrame* frame);
+int artfHipAdjoint(const ArtfHipAdjointArgs* args, void* stream);
+
 /* ---- the post passes: replace bloomDownRow/bloomUpRow (src/software/software.c:320-386, 419-426)
  * and lensFill's tone map (src/software/lens.inc:387-440), and the Vulkan shaders artfShaderBloomDown,
  * artfShaderBloomUp and artfShaderHoleBackdrop (src/vulkan/shaders.c:2058-2142, 2207-2257). ---- */

@@ -1,0 +1,158 @@
+"""The hand-written adjoint (artfHoleAdjointGradient) against central finite differences, in FP64 on the
+GPU and on the CPU oracle; its FP32 form against its FP64 form; its forward image against the tracer's;
+the torch.autograd.Function harness; a short fit; and its cost against the forward pass.
+
+Loss: MSE of the float32 HDR against a target rendered with hidden parameters (BASELINE config 5's
+scene, smaller). Parameters: spin, inclination (degrees), disc inner, disc outer, q. Finite differences
+of a pixelized image carry their own error (a pixel whose ray changes its fate between +h and -h adds
+a jump), so the tolerance is 2% of each component plus 0.1% of the largest."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from artful_amd import abi
+from tests import kerr_autograd as K
+
+pytestmark = pytest.mark.gpu
+
+W = H = 96
+STEPS = (1e-4, 1e-3, 1e-4, 1e-4, 1e-4)
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu):
+    c = gpu.HoleAdjoint(W, H)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def problem(ctx):
+    target = K.render(ctx, K.HIDDEN, "fp64").astype(np.float64)
+    p = np.array(K.START, np.float64)
+    img = K.render(ctx, p, "fp64").astype(np.float64)
+    seed = (2 * (img - target) / img.size).astype(np.float32)
+    return target, p, seed
+
+
+def close(g, want):
+    tol = .02 * np.abs(want) + 1e-3 * np.abs(want).max()
+    return np.all(np.abs(g - want) <= tol), np.abs(g - want) / np.maximum(np.abs(want), 1e-30)
+
+
+def test_forward_image_is_the_tracers(ctx, problem, gpu):
+    """The adjoint's own forward pass sees the image the tracer draws, bit for bit (both precisions),
+    and the renderer draws the same analytic disc from a scene."""
+    _, p, seed = problem
+    for precision in ("fp32", "fp64"):
+        _, hdr = K.gradient(ctx, p, seed, precision)
+        np.testing.assert_array_equal(hdr.view(np.uint32), K.render(ctx, p, precision).view(np.uint32))
+    desc, cam, q = K.scene(p)
+    s = gpu.Scene(desc, cam)
+    r = gpu.Renderer(W, H)
+    r.set_tracing("fp32", disc="analytic", emissivity=q, max_steps=2000, escape_radius=100.0)
+    assert r.draw(s)
+    np.testing.assert_array_equal(r.read_hdr().view(np.uint32), K.render(ctx, p).view(np.uint32))
+    r.close()
+
+
+def test_fp64_adjoint_against_gpu_differences(ctx, problem):
+    target, p, seed = problem
+    g, _ = K.gradient(ctx, p, seed, "fp64")
+    fd = []
+    for k, h in enumerate(STEPS):
+        e = np.zeros(5)
+        e[k] = h
+        lp = ((K.render(ctx, p + e, "fp64").astype(np.float64) - target) ** 2).mean()
+        lm = ((K.render(ctx, p - e, "fp64").astype(np.float64) - target) ** 2).mean()
+        fd.append((lp - lm) / (2 * h))
+    ok, rel = close(g, np.array(fd))
+    print("adjoint", g, "fd", fd, "rel", rel)
+    assert ok
+
+
+def test_fp64_adjoint_against_oracle_differences(gpu, oracle):
+    """Central differences of the CPU FP64 oracle (an independent tracer) against the GPU's FP64 adjoint."""
+    w = h = 48
+    c = gpu.HoleAdjoint(w, h)
+    themap = np.zeros(abi.MAP_FLOATS, np.float32)
+    themap[abi.COLORS_AT:abi.SKY_AT] = gpu.black_body_colors()
+    fluid = np.zeros(abi.FLUID_WIDTH * abi.FLUID_HEIGHT, np.float32)
+
+    def oracle_image(params):
+        desc, cam, q = K.scene(params)
+        f, f64 = gpu.frames_for(desc, cam, w, h)
+        lw = oracle.law(max_steps=2000, escape_radius=100.0, disc="analytic", emissivity=q)
+        return oracle.trace(f, themap, fluid, 64, f64, lw).astype(np.float64)
+
+    target = oracle_image(K.HIDDEN)
+    p = np.array(K.START, np.float64)
+    img = oracle_image(p)
+    seed = (2 * (img - target) / img.size).astype(np.float32)
+    g, _ = K.gradient(c, p, seed, "fp64")
+    fd = []
+    for k, hh in enumerate(STEPS):
+        e = np.zeros(5)
+        e[k] = hh
+        fd.append((((oracle_image(p + e) - target) ** 2).mean() - ((oracle_image(p - e) - target) ** 2).mean()) / (2 * hh))
+    ok, rel = close(g, np.array(fd))
+    print("adjoint", g, "oracle fd", fd, "rel", rel)
+    c.close()
+    assert ok
+
+
+def test_fp32_adjoint_against_fp64(ctx, problem):
+    _, p, seed = problem
+    g32, _ = K.gradient(ctx, p, seed, "fp32")
+    g64, _ = K.gradient(ctx, p, seed, "fp64")
+    assert np.all(np.abs(g32 - g64) <= 1e-3 * np.abs(g64) + 1e-4 * np.abs(g64).max())
+
+
+def test_linear_in_the_seed(ctx, problem):
+    _, p, seed = problem
+    g1, _ = K.gradient(ctx, p, seed, "fp64")
+    g2, _ = K.gradient(ctx, p, 2 * seed, "fp64")
+    np.testing.assert_allclose(g2, 2 * g1, rtol=1e-9, atol=1e-15)
+    g0, _ = K.gradient(ctx, p, np.zeros_like(seed), "fp64")
+    assert np.all(g0 == 0)
+
+
+def test_autograd_function(ctx, problem):
+    import torch
+    target, p, seed = problem
+    params = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    img = K.KerrImage.apply(params, ctx)
+    loss = K.mse(img.double(), torch.from_numpy(target))
+    loss.backward()
+    img32 = K.render(ctx, p).astype(np.float64)
+    direct, _ = K.gradient(ctx, p, (2 * (img32 - target) / img32.size).astype(np.float32))
+    np.testing.assert_allclose(params.grad.numpy(), direct, rtol=1e-5, atol=1e-12)
+
+
+def test_fit_moves_toward_hidden(ctx, problem):
+    """Adam on the five parameters from the start point: the loss falls by half in 40 steps and every
+    parameter but the least constrained moves toward its hidden value."""
+    target, _, _ = problem
+    history = K.fit(ctx, target, iterations=40)
+    first, last = history[0][0], history[-1][0]
+    print("loss", first, "->", last, "params", history[-1][1])
+    assert last < first / 2
+    moved = np.abs(history[-1][1] - np.array(K.HIDDEN)) < np.abs(np.array(K.START) - np.array(K.HIDDEN))
+    assert moved.sum() >= 4
+
+
+def test_adjoint_costs_at_most_four_forwards(gpu):
+    """BASELINE config 5 at full size: 512x512, one ray a pixel."""
+    c = gpu.HoleAdjoint(512, 512)
+    target = K.render(c, K.HIDDEN)
+    ratios = []
+    for _ in range(3):
+        img = K.render(c, K.START)
+        fwd, _ = c.times()
+        K.gradient(c, K.START, (2 * (img - target) / img.size).astype(np.float32))
+        _, adj = c.times()
+        ratios.append(adj / fwd)
+    c.close()
+    print("adjoint / forward", ratios)
+    assert min(ratios) <= 4.0
