@@ -120,6 +120,31 @@ def cpu_baseline(native, seconds=15.0):
             "sample": f"oracle restatement, {threads} threads: rows {band[0]}..{band[1]} of the same frame, 2x2 rays/pixel, trace only"}
 
 
+def adjoint_fit(native, iterations=200):
+    """512x512, one ray a pixel, the analytic disc; the adjoint's cost against the
+    forward trace, and a 200-step Adam fit of (spin, inclination, disc inner, outer, q) from the start
+    point to the image of the hidden parameters."""
+    from tests import kerr_autograd as K
+    ctx = native.HoleAdjoint(512, 512)
+    target = K.render(ctx, K.HIDDEN)
+    fwd, adj = [], []
+    for _ in range(5):
+        img = K.render(ctx, K.START)
+        fwd.append(ctx.times()[0])
+        K.gradient(ctx, K.START, (2 * (img - target) / img.size).astype(np.float32))
+        adj.append(ctx.times()[1])
+    t0 = time.perf_counter()
+    history = K.fit(ctx, target, iterations=iterations)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    best = min(range(len(history)), key=lambda k: history[k][0])
+    return {"forward_ms": float(np.median(fwd)), "adjoint_ms": float(np.median(adj)),
+            "adjoint_over_forward": float(np.median(adj) / np.median(fwd)),
+            "fit": {"iterations": iterations, "wall_s": wall, "loss_first": history[0][0], "loss_last": history[-1][0],
+                    "loss_best": history[best][0], "params_last": [float(v) for v in history[-1][1]],
+                    "hidden": list(K.HIDDEN), "start": list(K.START)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +199,8 @@ def main():
     achieved = flops / (fp32["trace_ms"] * 1e-3) / 1e12
     traffic = traffic_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(native)
+    adjoint = adjoint_fit(native)
+    log(f"adjoint: {adjoint}")
 
     line = {
         "metric": "Kerr rays/s at 1080p (FP32 vs FP64 plain, <=1e-3 rel err); adjoint/forward time",
@@ -192,7 +219,8 @@ def main():
         "trace_ms": fp32["trace_ms"], "post_ms": fp32["post_ms"],
         "steps_per_ray": fp32["steps_per_frame"] / rays, "active_lane_fraction": fp32["active_lanes"],
         "rel_error_vs_fp64": errors,
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": adjoint["adjoint_over_forward"],
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 
