@@ -94,6 +94,35 @@ class HipAdjointArgs(C.Structure):
     _fields_ = [("trace", HipTraceArgs), ("seed", C.c_void_p * 3), ("partials", C.c_void_p)]
 
 
+class HipView(C.Structure):
+    _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("rings", C.c_void_p), ("glow", C.c_void_p),
+                ("fluid", C.c_void_p), ("out", C.c_void_p * 3), ("maxSteps", C.c_int32), ("escapeRadius", C.c_float),
+                ("disc", C.c_int32), ("emissivity", C.c_float), ("jitter", C.c_uint32), ("tilesX", C.c_uint32),
+                ("tilesY", C.c_uint32), ("rays", C.c_uint32)]
+
+
+class HipSchedule(C.Structure):
+    _fields_ = [("workgroups", C.c_int32), ("chunkTiles", C.c_int32), ("refill", C.c_int32), ("stepsPerCheck", C.c_int32)]
+
+
+class HipBatchArgs(C.Structure):
+    _fields_ = [("views", C.c_void_p), ("firstTile", C.c_void_p), ("viewCount", C.c_int32), ("tiles", C.c_uint32),
+                ("precision", C.c_int32), ("map", C.c_void_p), ("queue", C.c_void_p), ("counters", C.c_void_p),
+                ("viewDone", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule)]
+
+
+class HoleView(C.Structure):
+    _fields_ = [("hole", BlackHoleDesc), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32)]
+
+
+HoleSchedule = HipSchedule  # the same four fields (include/artful/batch.h)
+
+
+class HoleBatchStats(C.Structure):
+    _fields_ = [("batchMs", C.c_double), ("traceMs", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64),
+                ("laneSlots", C.c_uint64), ("views", C.c_int32), ("schedule", HipSchedule)]
+
+
 # enum values (include/artful/renderer.h, include/artful/hole.h)
 BACKEND_AUTO, BACKEND_VULKAN, BACKEND_SOFTWARE, BACKEND_METAL, BACKEND_HIP = range(5)
 PRECISION = {"fp32": 0, "fp32_fast": 1, "mixed": 2, "fp64": 3}
@@ -113,3 +142,4 @@ DUST_AT = GLOW_AT + MAP_HEIGHT
 MAP_FLOATS = DUST_AT + DUST_WIDTH * DUST_HEIGHT
 
 assert C.sizeof(BlackHoleDesc) == 92 and C.sizeof(Camera) == 44 and C.sizeof(HoleFrame) == 368
+assert C.sizeof(HipView) == 800 and C.sizeof(HoleView) == 144

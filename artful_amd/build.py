@@ -25,7 +25,7 @@ OBJ = ROOT / "_build" / "artful_hip"
 ARCH = os.environ.get("ARTF_OFFLOAD_ARCH", "gfx950")
 
 C_SOURCES = ["core/base.c", "core/pool.c", "scene/disc.c", "scene/fluid.c", "scene/hole.c", "scene/scene.c",
-             "render/renderer.c", "render/hip_backend.c", "render/hole_adjoint.c"]
+             "render/renderer.c", "render/hip_backend.c", "render/hole_adjoint.c", "render/hole_batch.c"]
 # (source, contraction)
 HIP_SOURCES = [("hip/trace_exact.hip", "off"), ("hip/trace_fast.hip", "fast"), ("hip/post.hip", "off"),
                ("hip/adjoint.hip", "off"), ("hip/abi.hip", "off")]

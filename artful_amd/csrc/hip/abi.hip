@@ -136,4 +136,63 @@ int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const blo
     return (int)launchToneMap(*f, hdr, bloom, bgra, (hipStream_t)stream);
 }
 
+static bool batchFits(const ArtfHipBatchArgs* a) {
+    if (!a || !a->views || !a->firstTile || !a->map || !a->queue || a->viewCount < 1 || a->tiles < 1 || a->tiles > (1u << 25)) return false;
+    if ((a->viewDone == NULL) != (a->stamps == NULL)) return false;
+    const ArtfHipSchedule& s = a->schedule;
+    if (s.workgroups < 1 || s.workgroups > 65536 || s.chunkTiles < 1 || s.chunkTiles > 64 || s.refill < 1 || s.refill > 64 || s.stepsPerCheck < 1 ||
+        s.stepsPerCheck > 16)
+        return false;
+    return a->precision >= ARTF_PRECISION_FP32 && a->precision <= ARTF_PRECISION_FP64;
+}
+
+int artfHipBatchReset(const ArtfHipBatchArgs* a, void* stream) {
+    if (!batchFits(a)) return (int)hipErrorInvalidValue;
+    return (int)launchBatchReset(*a, (hipStream_t)stream);
+}
+
+/* The views themselves live on the device, where the host cannot check them: the caller (the batch in
+ * render/hole_batch.c) builds them from frames it has checked with frameFits and uploads them. */
+int artfHipTraceViews(const ArtfHipBatchArgs* a, void* stream) {
+    if (!batchFits(a)) return (int)hipErrorInvalidValue;
+    if (a->precision == ARTF_PRECISION_FP32) return (int)launchViewsExact(*a, (hipStream_t)stream);
+    return (int)launchViewsFast(*a, (hipStream_t)stream);
+}
+
+int artfHipViewsOccupancy(int precision) {
+    if (precision == ARTF_PRECISION_FP32) return viewsOccupancyExact();
+    return viewsOccupancyFast(precision);
+}
+
+double artfHipWallClockHz(int device) {
+    int kHz = 0;
+    if (hipDeviceGetAttribute(&kHz, hipDeviceAttributeWallClockRate, device) != hipSuccess) return 0.0;
+    return (double)kHz * 1e3;
+}
+
+int artfHipGraphBegin(void* stream) { return (int)hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal); }
+
+int artfHipGraphEnd(void* stream, void** graph) {
+    if (!graph) return (int)hipErrorInvalidValue;
+    *graph = NULL;
+    hipGraph_t g = NULL;
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+    if (e != hipSuccess) return (int)e;
+    hipGraphExec_t exec = NULL;
+    e = hipGraphInstantiate(&exec, g, NULL, NULL, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return (int)e;
+    *graph = (void*)exec;
+    return 0;
+}
+
+int artfHipGraphLaunch(void* graph, void* stream) {
+    if (!graph) return (int)hipErrorInvalidValue;
+    return (int)hipGraphLaunch((hipGraphExec_t)graph, (hipStream_t)stream);
+}
+
+void artfHipGraphFree(void* graph) {
+    if (graph) (void)hipGraphExecDestroy((hipGraphExec_t)graph);
+}
+
 } // extern "C"

@@ -16,6 +16,14 @@ constexpr int kGroup = 256;
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
 /* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */
 hipError_t launchTraceFast(const ArtfHipTraceArgs& a, hipStream_t stream);
+/* The persistent tracer of a batch of views: FP32 (trace_exact.hip), the others (trace_fast.hip); and
+ * the workgroups of it a compute unit holds. */
+hipError_t launchViewsExact(const ArtfHipBatchArgs& a, hipStream_t stream);
+hipError_t launchViewsFast(const ArtfHipBatchArgs& a, hipStream_t stream);
+int viewsOccupancyExact();
+int viewsOccupancyFast(int precision);
+/* Clears a batch's workspace and stamps its start (post.hip). */
+hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream);
 /* The analytic-disc adjoint (adjoint.hip, contraction off). */
 hipError_t launchAdjoint(const ArtfHipAdjointArgs& a, hipStream_t stream);
 int adjointBlocks(const ArtfHoleFrame& f);

@@ -65,7 +65,7 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
         StepLooks<double> l;
         nystromStep(cd, w, m, l);
         if (w.n2 * m.n2 < 0.0) {
-            if (law.disc == DISC_PAINTED) crossPainted(f, d, cd, w, m, l, map, fluid, acc);
+            if (law.disc == DISC_PAINTED) crossPainted(d, cd, w, m, l, tablesOf(map, fluid), acc);
             else crossAnalytic(d, cd, w, m, l, law.q, map, acc);
         }
         const bool escaped = uEscape > m.u;
@@ -83,7 +83,7 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
         }
         s = castState<float>(m);
     }
-    if (f.look[2] > 0.f && sky) skyLight(f, cd, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
+    if (f.look[2] > 0.f && sky) skyLight(f.look[2], cd, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
     RayResult out;
     out.r = (float)acc.r, out.g = (float)acc.g, out.b = (float)acc.b, out.steps = steps;
     return out;

@@ -24,6 +24,24 @@ __device__ __forceinline__ void subPixel(uint32_t seed, int px, int py, int sx, 
     dy = ((float)sy + jy) / (float)side;
 }
 
+/* The moment of sub-ray sub of side^2 within the shutter (lens.inc:371). */
+__device__ __forceinline__ float rayMoment(const ArtfHoleFrame& f, int sub, int side) {
+    const float moment = ((float)((sub * 5) % (side * side)) + .5f) * f.image[3] - .5f;
+    return f.disc[2] + f.spin[3] * moment;
+}
+
+/* A ray's place in -1..1 across the image: in FP32 as lensTile forms it (the pixel's group of 8,
+ * plus the offset, then the lane), in FP64 directly. */
+__device__ __forceinline__ void aimOf(const ArtfHoleFrame& f, int px, int py, float dx, float dy, float& nx, float& ny) {
+    nx = ((float)(px & 7) + ((float)(px & ~7) + dx)) * f.image[1] - 1.f;
+    ny = 1.f - ((float)py + dy) * f.image[2];
+}
+
+__device__ __forceinline__ void aimOf(const ArtfHoleFrame& f, int px, int py, float dx, float dy, double& nx, double& ny) {
+    nx = ((double)px + (double)dx) * (2.0 / (double)f.size[0]) - 1.0;
+    ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)f.size[1]);
+}
+
 /* The wave's step count and its lane-slots (its longest ray times its lanes with a pixel), added by
  * one lane to the counter slot of its workgroup (slots spread the atomics over many addresses). */
 __device__ __forceinline__ void countSteps(unsigned long long* counters, int steps, bool valid) {
@@ -63,24 +81,22 @@ __global__ __launch_bounds__(kGroup) void traceKernel(ArtfHipTraceArgs a) {
     if (valid) {
         float dx, dy;
         subPixel(a.jitter, px, py, sx, sy, side, dx, dy);
-        const float moment = ((float)((sub * 5) % (side * side)) + .5f) * f.image[3] - .5f;
-        const float time = f.disc[2] + f.spin[3] * moment;
+        const float time = rayMoment(f, sub, side);
         if (Precision == ARTF_PRECISION_FP64) {
             Eye<double> e;
             eyeFrom64(a.frame64, e);
-            const double nx = ((double)px + (double)dx) * (2.0 / (double)width) - 1.0;
-            const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
+            double nx, ny;
+            aimOf(f, px, py, dx, dy, nx, ny);
             ray = traceRay<double>(f, e, law, a.map, a.fluid, nx, ny, time);
         } else if (Precision == ARTF_PRECISION_MIXED) {
-            const double nx = ((double)px + (double)dx) * (2.0 / (double)width) - 1.0;
-            const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
+            double nx, ny;
+            aimOf(f, px, py, dx, dy, nx, ny);
             ray = traceRayMixed(f, a.frame64, law, a.map, a.fluid, nx, ny, time);
         } else {
             Eye<float> e;
             eyeFrom<float>(f, e);
-            /* (as lensTile forms it: the pixel's group of 8, plus the offset, then the lane) */
-            const float nx = ((float)(px & 7) + ((float)(px & ~7) + dx)) * f.image[1] - 1.f;
-            const float ny = 1.f - ((float)py + dy) * f.image[2];
+            float nx, ny;
+            aimOf(f, px, py, dx, dy, nx, ny);
             ray = traceRay<float>(f, e, law, a.map, a.fluid, nx, ny, time);
         }
     }

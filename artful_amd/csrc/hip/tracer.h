@@ -195,6 +195,22 @@ template <typename R> struct Gathered {
     R r, g, b, through;
 };
 
+/* Where the painted disc's tables are: the map's shared part (gas, colours, sky, dust), and the ring
+ * table (speed, heat per ring), glow per ring and fluid field of one view's disc. A scene's map holds
+ * its own rings and glow (tablesOf); a batch of views keeps them apart, one set per view. */
+struct DiscTables {
+    const float* map;
+    const float* rings;
+    const float* glow;
+    const float* fluid;
+};
+
+__device__ __forceinline__ DiscTables tablesOf(const float* map, const float* fluid) {
+    DiscTables t;
+    t.map = map, t.rings = map + ARTF_HOLE_ROWS_AT, t.glow = map + ARTF_HOLE_GLOW_AT, t.fluid = fluid;
+    return t;
+}
+
 template <typename R> __device__ __forceinline__ R bendOf(const RayConst<R>& c, R x) { return x * (c.uA + x * (c.uB + x * c.uC)); }
 template <typename R> __device__ __forceinline__ R dragOf(const RayConst<R>& c, R x) {
     return quot(c.minusA * x * ((R)2 - c.xiA * x), hi((R)1 - (x + x) + c.a2 * x * x, (R)1e-3f));
@@ -306,17 +322,17 @@ template <typename R> __device__ __forceinline__ R orbitShift(const RayConst<R>&
 /* The painted disc where the ray crosses its plane in this step (lens.inc:195-291): three slices
  * through the disc's depth, front to back, then black-body colour at the shifted heat. */
 template <typename R>
-__device__ __forceinline__ void crossPainted(const ArtfHoleFrame& f, const DiscConst<R>& d, const RayConst<R>& c, const RayState<R>& s,
-                                             const RayState<R>& n, const StepLooks<R>& l, const float* __restrict__ map,
-                                             const float* __restrict__ fluid, Gathered<R>& acc) {
+__device__ __forceinline__ void crossPainted(const DiscConst<R>& d, const RayConst<R>& c, const RayState<R>& s, const RayState<R>& n,
+                                             const StepLooks<R>& l, const DiscTables& tables, Gathered<R>& acc) {
     const R zero = 0, one = 1;
     const R part = quot(s.n2, s.n2 - n.n2);
     const R hu = part * c.step, hh = hu * hu * (R).5f;
     const R uc = s.u + hu * s.w + hh * l.k1;
     const R rhoCenter = (one - d.inner * uc) * d.rhoScale;
     if (!(rhoCenter > (R)-.03f && rhoCenter < (R)1.02f)) return;
-    const float* rows = map + ARTF_HOLE_ROWS_AT;
-    const float* colors = map + ARTF_HOLE_COLORS_AT;
+    const float* __restrict__ map = tables.map;
+    const float* __restrict__ rows = tables.rings;
+    const float* __restrict__ colors = map + ARTF_HOLE_COLORS_AT;
     const R wc = s.w + hu * l.k1;
     const R cx = s.n0 + hu * s.v0 + hh * l.a1x, cy = s.n1 + hu * s.v1 + hh * l.a1y;
     const R vx = s.v0 + hu * l.a1x, vy = s.v1 + hu * l.a1y, vz = s.v2 + hu * l.a1z;
@@ -351,7 +367,7 @@ __device__ __forceinline__ void crossPainted(const ArtfHoleFrame& f, const DiscC
         const R fluidFloor = lo(floorOf(fluidAt), (R)(float)(ARTF_FLUID_HEIGHT - 2));
         const int32_t fluidRow = toInt(fluidFloor);
         const R fluidTurn = turnK - speed0 * d.flowLag;
-        const R q0 = rowAt(fluid, ARTF_FLUID_WIDTH, fluidRow, fluidTurn), q1 = rowAt(fluid, ARTF_FLUID_WIDTH, fluidRow + 1, fluidTurn);
+        const R q0 = rowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow, fluidTurn), q1 = rowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow + 1, fluidTurn);
         const R gas = q0 + (q1 - q0) * (fluidAt - fluidFloor);
         density = density * ((R)(1.f - ARTF_HOLE_FLUID) + (R)ARTF_HOLE_FLUID * gas);
         const R dustAt = rhoK * (R)(float)(ARTF_HOLE_DUST_HEIGHT - 1);
@@ -370,7 +386,7 @@ __device__ __forceinline__ void crossPainted(const ArtfHoleFrame& f, const DiscC
     const R rhoSafe = lo(hi(rhoCenter, zero), one);
     const int32_t rowIndex = toInt(rhoSafe * (R)(float)(ARTF_HOLE_MAP_HEIGHT - 1));
     R heat = (R)rows[rowIndex + rowIndex + 1];
-    const R gives = (R)map[ARTF_HOLE_GLOW_AT + rowIndex];
+    const R gives = (R)tables.glow[rowIndex];
     R light = d.gain * glow * gives;
     if (d.doppler != zero) {
         const R g = orbitShift(c, uc, d.doppler);
@@ -423,7 +439,7 @@ __device__ __forceinline__ void crossAnalytic(const DiscConst<R>& d, const RayCo
 
 /* Stars and the sky's dust toward where the ray left, rotated by its drag (lens.inc:316-347). */
 template <typename R>
-__device__ __forceinline__ void skyLight(const ArtfHoleFrame& f, const RayConst<R>& c, R awayX, R awayY, R awayZ, R awayPsi, R doppler,
+__device__ __forceinline__ void skyLight(float stars, const RayConst<R>& c, R awayX, R awayY, R awayZ, R awayPsi, R doppler,
                                          const float* __restrict__ map, Gathered<R>& acc) {
     const R zero = 0, one = 1;
     const R invAway = quot(one, squareRoot(hi(awayX * awayX + awayY * awayY + awayZ * awayZ, (R)1e-12f)));
@@ -444,7 +460,7 @@ __device__ __forceinline__ void skyLight(const ArtfHoleFrame& f, const RayConst<
     r = r + dust * (R).92f;
     g = g + dust * (R).86f;
     b = b + dust;
-    R shown = acc.through * (R)f.look[2];
+    R shown = acc.through * (R)stars;
     if (doppler != zero) {
         const R g1 = one + doppler * (c.invEnergy * c.moved - one);
         const R g2 = g1 * g1;
@@ -455,12 +471,19 @@ __device__ __forceinline__ void skyLight(const ArtfHoleFrame& f, const RayConst<
     acc.b = acc.b + b * shown;
 }
 
-template <typename R> __device__ __forceinline__ void discFrom(const ArtfHoleFrame& f, const Eye<R>& e, float time, DiscConst<R>& d) {
-    d.inner = e.inner, d.outer = e.outer, d.thickness = e.thickness, d.discHeight = e.discHeight;
-    d.rhoScale = quot((R)1, (R)1 - quot(e.inner, e.outer));
+/* The disc's constants for a ray at this moment; inner, outer, thickness and height in R (from the
+ * FP32 frame, or the FP64 one). */
+template <typename R>
+__device__ __forceinline__ void discOf(const ArtfHoleFrame& f, R inner, R outer, R thickness, R discHeight, float time, DiscConst<R>& d) {
+    d.inner = inner, d.outer = outer, d.thickness = thickness, d.discHeight = discHeight;
+    d.rhoScale = quot((R)1, (R)1 - quot(inner, outer));
     d.slant3 = (R)(f.disc[3] / (float)ARTF_LENS_SLICES);
     d.gain = (R)f.look[0], d.doppler = (R)f.look[1];
     d.time = (R)time, d.flowLag = (R)(time - f.flow[0]);
+}
+
+template <typename R> __device__ __forceinline__ void discFrom(const ArtfHoleFrame& f, const Eye<R>& e, float time, DiscConst<R>& d) {
+    discOf(f, e.inner, e.outer, e.thickness, e.discHeight, time, d);
 }
 
 /* One ray, eye to end, all in R. nx, ny: the pixel's place in -1..1; time: this ray's moment. */
@@ -482,7 +505,7 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
         StepLooks<R> looks;
         nystromStep(c, s, n, looks);
         if (s.n2 * n.n2 < (R)0) { /* through the disc's plane */
-            if (law.disc == DISC_PAINTED) crossPainted(f, d, c, s, n, looks, map, fluid, acc);
+            if (law.disc == DISC_PAINTED) crossPainted(d, c, s, n, looks, tablesOf(map, fluid), acc);
             else crossAnalytic(d, c, s, n, looks, law.q, map, acc);
         }
         const bool escaped = uEscape > n.u;
@@ -500,7 +523,7 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
         }
         s = n;
     }
-    if (f.look[2] > 0.f && sky) skyLight(f, c, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
+    if (f.look[2] > 0.f && sky) skyLight(f.look[2], c, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
     RayResult out;
     out.r = (float)acc.r, out.g = (float)acc.g, out.b = (float)acc.b, out.steps = steps;
     return out;

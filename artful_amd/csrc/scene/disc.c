@@ -175,9 +175,7 @@ void artfDiscPaint(float* map) {
 
 /* Per ring (rows even in 1/r between inner and outer): turns per unit time of a Kerr circular orbit
  * 1/(2 pi (r^1.5 + a)), the one heat of 4500 K in table units, and light 6 (r/inner)^1.5. */
-void artfDiscRings(float* map, float inner, float outer, float spin) {
-    float* rows = map + ARTF_HOLE_ROWS_AT;
-    float* glow = map + ARTF_HOLE_GLOW_AT;
+void artfDiscRingTables(float* rows, float* glow, float inner, float outer, float spin) {
     for (int y = 0; y < ARTF_HOLE_MAP_HEIGHT; ++y) {
         const float rho = (float)y / (float)(ARTF_HOLE_MAP_HEIGHT - 1);
         const float r = inner / (1.f - rho * (1.f - inner / outer));
@@ -187,3 +185,5 @@ void artfDiscRings(float* map, float inner, float outer, float spin) {
         glow[y] = 6.f * out * sqrtf(out);
     }
 }
+
+void artfDiscRings(float* map, float inner, float outer, float spin) { artfDiscRingTables(map + ARTF_HOLE_ROWS_AT, map + ARTF_HOLE_GLOW_AT, inner, outer, spin); }

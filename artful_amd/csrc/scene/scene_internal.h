@@ -27,6 +27,8 @@ struct ArtfScene {
 /* ---- disc.c: the painted gas and the per-ring tables (Artful src/scene/hole.c:10-197) ---- */
 void artfDiscPaint(float* map);  /* gas, dust, colours, sky: once per map */
 void artfDiscRings(float* map, float inner, float outer, float spin);
+/* The same tables apart from a map: rows ARTF_HOLE_MAP_HEIGHT x 2 (speed, heat), glow ARTF_HOLE_MAP_HEIGHT. */
+void artfDiscRingTables(float* rows, float* glow, float inner, float outer, float spin);
 
 /* ---- fluid.c (Artful src/scene/fluid.c) ---- */
 ArtfFluid* artfFluidNew(float inner, float outer, float spin, double time);

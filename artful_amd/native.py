@@ -71,6 +71,18 @@ def lib() -> C.CDLL:
         "artfHoleAdjointGradient": ([P, C.POINTER(abi.BlackHoleDesc), C.POINTER(abi.Camera), C.POINTER(abi.HoleTracing), P, P,
                                      C.POINTER(abi.HoleGradient)], B),
         "artfHoleAdjointTimes": ([P, C.POINTER(D), C.POINTER(D)], None),
+        "artfHipBatchReset": ([C.POINTER(abi.HipBatchArgs), P], I),
+        "artfHipTraceViews": ([C.POINTER(abi.HipBatchArgs), P], I),
+        "artfHipViewsOccupancy": ([I], I), "artfHipWallClockHz": ([I], D),
+        "artfHipGraphBegin": ([P], I), "artfHipGraphEnd": ([P, C.POINTER(P)], I),
+        "artfHipGraphLaunch": ([P, P], I), "artfHipGraphFree": ([P], None),
+        "artfHoleBatchNew": ([C.POINTER(abi.HoleView), I, C.POINTER(abi.HoleTracing), I], P),
+        "artfHoleBatchFree": ([P], None), "artfHoleBatchDraw": ([P], B),
+        "artfHoleBatchReadHdr": ([P, I, P], B),
+        "artfHoleBatchStats": ([P, C.POINTER(abi.HoleBatchStats)], B),
+        "artfHoleBatchLatency": ([P, P], B),
+        "artfHoleBatchSetSchedule": ([P, C.POINTER(abi.HoleSchedule)], B),
+        "artfHoleBatchTune": ([P, I, C.POINTER(abi.HoleSchedule)], B),
     }
     for name, (args, ret) in sig.items():
         fn = getattr(L, name)
@@ -323,3 +335,65 @@ class HoleAdjoint:
 def calls_left(n: int) -> None:
     """Test hook: device calls that succeed before the device is reported lost (-1: no limit)."""
     lib().artfHipCallsLeft.value = n
+
+
+class HoleBatch:
+    """Many views traced by one persistent launch (include/artful/batch.h). views: (desc, camera, width,
+    height) tuples; tracing: a HoleTracing (artful_amd.native.tracing) for all of them."""
+
+    def __init__(self, views, trc=None, device: int = 0):
+        self._L = lib()
+        arr = (abi.HoleView * len(views))()
+        self.sizes = []
+        for i, (desc, cam, w, h) in enumerate(views):
+            arr[i].hole, arr[i].camera, arr[i].width, arr[i].height = desc, cam, w, h
+            self.sizes.append((w, h))
+        self.ptr = self._L.artfHoleBatchNew(arr, len(views), C.byref(trc) if trc is not None else None, device)
+        if not self.ptr:
+            raise RuntimeError("artfHoleBatchNew failed")
+
+    def draw(self) -> None:
+        if not self._L.artfHoleBatchDraw(self.ptr):
+            raise RuntimeError("artfHoleBatchDraw failed")
+
+    def read_hdr(self, view: int) -> np.ndarray:
+        w, h = self.sizes[view]
+        out = np.zeros((3, h, w), np.float32)
+        if not self._L.artfHoleBatchReadHdr(self.ptr, view, out.ctypes.data):
+            raise RuntimeError("artfHoleBatchReadHdr failed")
+        return out
+
+    def stats(self) -> abi.HoleBatchStats:
+        s = abi.HoleBatchStats()
+        if not self._L.artfHoleBatchStats(self.ptr, C.byref(s)):
+            raise RuntimeError("artfHoleBatchStats failed")
+        return s
+
+    def latency(self) -> np.ndarray:
+        out = np.zeros(len(self.sizes), np.float64)
+        if not self._L.artfHoleBatchLatency(self.ptr, out.ctypes.data):
+            raise RuntimeError("artfHoleBatchLatency failed")
+        return out
+
+    def set_schedule(self, workgroups=0, chunk_tiles=0, refill=0, steps_per_check=0) -> None:
+        s = abi.HoleSchedule(workgroups, chunk_tiles, refill, steps_per_check)
+        if not self._L.artfHoleBatchSetSchedule(self.ptr, C.byref(s)):
+            raise RuntimeError("artfHoleBatchSetSchedule failed")
+
+    def tune(self, rounds: int = 0) -> abi.HoleSchedule:
+        s = abi.HoleSchedule()
+        if not self._L.artfHoleBatchTune(self.ptr, rounds, C.byref(s)):
+            raise RuntimeError("artfHoleBatchTune failed")
+        return s
+
+    def close(self) -> None:
+        if self.ptr:
+            self._L.artfHoleBatchFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -76,6 +76,75 @@ int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
  * does (lens.inc:364-380). rays: the planes artfHipTrace wrote; hdr: width * height each. */
 int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* const hdr[3], void* stream);
 
+/* ---- batches of views (not in the reference, which draws one view a frame): many views of any
+ * size, spin, eye and disc, traced by one persistent launch whose waves take 8x8 tiles of rays from a
+ * queue and give each lane a new ray as soon as its last one ends. Each view's rays are those
+ * artfHipTrace would trace for its frame, bit for bit in FP32. ---- */
+
+typedef struct ArtfHipView {
+    ArtfHoleFrame frame;     /* artfHoleFrame's FP32 uniforms for this view */
+    ArtfHoleFrame64 frame64; /* read by the MIXED and FP64 tracers */
+    /* device tables of this view's disc: ring speed and heat (ARTF_HOLE_MAP_HEIGHT x 2, as the map's
+     * rows part at ARTF_HOLE_ROWS_AT), glow per ring (ARTF_HOLE_MAP_HEIGHT, as at ARTF_HOLE_GLOW_AT),
+     * the fluid field (ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT; read by the painted disc only) */
+    const float* rings;
+    const float* glow;
+    const float* fluid;
+    float* out[3];        /* device planes, as artfHipTrace's out[] for this view's frame */
+    int32_t maxSteps;     /* at least 1 */
+    float escapeRadius;   /* 0: where 1/r crosses 0 */
+    int32_t disc;         /* ArtfDiscModel */
+    float emissivity;     /* the analytic disc's q */
+    uint32_t jitter;      /* 0: the stratified grid; else a seed */
+    uint32_t tilesX, tilesY; /* (width + 7) / 8, (height + 7) / 8 */
+    uint32_t rays;        /* width * height * side^2 */
+} ArtfHipView;
+
+/* How the persistent launch spends the device. */
+typedef struct ArtfHipSchedule {
+    int32_t workgroups;    /* of 256 lanes (four independent waves); at least 1 */
+    int32_t chunkTiles;    /* 8x8 tiles of rays a wave takes from the queue at once, 1..64 */
+    int32_t refill;        /* a wave gives new rays to its idle lanes once at least this many of its 64 are idle, 1..64 */
+    int32_t stepsPerCheck; /* steps between two looks at the idle lanes, 1..16 */
+} ArtfHipSchedule;
+
+typedef struct ArtfHipBatchArgs {
+    const ArtfHipView* views;  /* device, viewCount */
+    const uint32_t* firstTile; /* device, viewCount + 1: view v's tiles are [firstTile[v], firstTile[v + 1]), each
+                                * view's tiles sub-ray by sub-ray, rows of tiles top to bottom */
+    int32_t viewCount;
+    uint32_t tiles;            /* firstTile[viewCount] */
+    int32_t precision;         /* ArtfPrecision, the same for every view */
+    const float* map;          /* device, ARTF_HOLE_MAP_FLOATS: its gas, colours, sky and dust are shared */
+    /* device workspace, cleared by artfHipBatchReset before each launch: the queue's next tile (1 word) */
+    uint32_t* queue;
+    /* device, 2 * ARTF_HIP_COUNTER_SLOTS, as artfHipTrace's counters (lane-slots: every lane of every
+     * stepping wave, idle ones included); NULL: not counted */
+    unsigned long long* counters;
+    /* device, viewCount words: rays of each view done so far; and stamps, viewCount + 1 words: the
+     * launch's start (stamps[0]) and the moment each view's last ray ended (stamps[1 + v]), in the
+     * device's wall clock (artfHipWallClockHz). Both NULL: not kept. */
+    uint32_t* viewDone;
+    unsigned long long* stamps;
+    ArtfHipSchedule schedule;
+} ArtfHipBatchArgs;
+
+/* Clears the queue, the counters, viewDone and the stamps, and stamps the start, with a kernel (no
+ * memset node, so that the pair can be captured in a graph and replayed). */
+int artfHipBatchReset(const ArtfHipBatchArgs* args, void* stream);
+int artfHipTraceViews(const ArtfHipBatchArgs* args, void* stream);
+/* Workgroups of the persistent kernel for this precision one compute unit holds at once. */
+int artfHipViewsOccupancy(int precision);
+/* The frequency of the clock the stamps count, in Hz (0 if unknown). */
+double artfHipWallClockHz(int device);
+
+/* ---- graphs: what a stream does between Begin and End, replayed as one launch ---- */
+int artfHipGraphBegin(void* stream);
+/* *graph: an executable graph, to launch with artfHipGraphLaunch and free with artfHipGraphFree. */
+int artfHipGraphEnd(void* stream, void** graph);
+int artfHipGraphLaunch(void* graph, void* stream);
+void artfHipGraphFree(void* graph);
+
 /* ---- the adjoint (not in the reference): image-loss gradients of the analytic-disc tracer ---- */
 
 typedef struct ArtfHipAdjointArgs {

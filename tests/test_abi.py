@@ -28,6 +28,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
     assert C.sizeof(abi.HipTraceArgs) == 800
+    assert C.sizeof(abi.HipView) == 800 and C.sizeof(abi.HipBatchArgs) == 88 and C.sizeof(abi.HoleBatchStats) == 64
     assert abi.MAP_FLOATS == 2887424
 
 
@@ -62,6 +63,37 @@ def test_trace_rejects_bad_arguments(native):
     P3 = C.c_void_p * 3
     assert L.artfHipBloom(C.byref(f), P3(1, 1, 1), P3(1, 1, 1), None) != 0  # level table not artfHoleFrame's
     assert L.artfHipToneMap(C.byref(f), P3(1, 1, 1), P3(1, 1, 1), None, None) != 0
+
+
+def test_batch_rejects_bad_arguments(native):
+    """The batch's kernel entries check their schedule and buffers on the host; artfHoleBatchNew checks
+    its views and tracing before it looks for a device."""
+    L = native.lib()
+    a = abi.HipBatchArgs()
+    assert L.artfHipTraceViews(C.byref(a), None) != 0 and L.artfHipBatchReset(C.byref(a), None) != 0
+    a.views = a.firstTile = a.map = a.queue = 64
+    a.viewCount, a.tiles, a.precision = 1, 1, 0
+    for bad in [(0, 1, 1, 1), (1, 0, 1, 1), (1, 65, 1, 1), (1, 1, 0, 1), (1, 1, 65, 1), (1, 1, 1, 0), (1, 1, 1, 17)]:
+        a.schedule = abi.HipSchedule(*bad)
+        assert L.artfHipTraceViews(C.byref(a), None) != 0, bad
+    a.schedule = abi.HipSchedule(1, 1, 1, 1)
+    a.viewDone = 64  # viewDone without stamps
+    assert L.artfHipTraceViews(C.byref(a), None) != 0
+    a.viewDone, a.precision = None, 4
+    assert L.artfHipTraceViews(C.byref(a), None) != 0
+
+    desc, cam = native.hole_desc(), native.camera((0, 10, 50))
+    views = (abi.HoleView * 2)()
+    for v in views:
+        v.hole, v.camera, v.width, v.height = desc, cam, 16, 16
+    t = L.artfHoleTracingDefaults()
+    assert not L.artfHoleBatchNew(views, 0, C.byref(t), 0)
+    views[1].width = 0
+    assert not L.artfHoleBatchNew(views, 2, C.byref(t), 0)
+    views[1].width, t.precision = 16, 7
+    assert not L.artfHoleBatchNew(views, 2, C.byref(t), 0)
+    t.precision, t.maxSteps = 0, -1
+    assert not L.artfHoleBatchNew(views, 2, C.byref(t), 0)
 
 
 def test_scene_lifecycle(native):
