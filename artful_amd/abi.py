@@ -95,8 +95,7 @@ class HipAdjointArgs(C.Structure):
 
 
 class HipView(C.Structure):
-    _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("rings", C.c_void_p), ("glow", C.c_void_p),
-                ("fluid", C.c_void_p), ("out", C.c_void_p * 3), ("maxSteps", C.c_int32), ("escapeRadius", C.c_float),
+    _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("out", C.c_void_p * 3), ("maxSteps", C.c_int32), ("escapeRadius", C.c_float),
                 ("disc", C.c_int32), ("emissivity", C.c_float), ("jitter", C.c_uint32), ("tilesX", C.c_uint32),
                 ("tilesY", C.c_uint32), ("rays", C.c_uint32)]
 
@@ -107,8 +106,8 @@ class HipSchedule(C.Structure):
 
 class HipBatchArgs(C.Structure):
     _fields_ = [("views", C.c_void_p), ("firstTile", C.c_void_p), ("viewCount", C.c_int32), ("tiles", C.c_uint32),
-                ("precision", C.c_int32), ("map", C.c_void_p), ("queue", C.c_void_p), ("counters", C.c_void_p),
-                ("viewDone", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule)]
+                ("precision", C.c_int32), ("map", C.c_void_p), ("rings", C.c_void_p), ("fluids", C.c_void_p),
+                ("queue", C.c_void_p), ("counters", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule)]
 
 
 class HoleView(C.Structure):
@@ -142,4 +141,4 @@ DUST_AT = GLOW_AT + MAP_HEIGHT
 MAP_FLOATS = DUST_AT + DUST_WIDTH * DUST_HEIGHT
 
 assert C.sizeof(BlackHoleDesc) == 92 and C.sizeof(Camera) == 44 and C.sizeof(HoleFrame) == 368
-assert C.sizeof(HipView) == 800 and C.sizeof(HoleView) == 144
+assert C.sizeof(HipView) == 776 and C.sizeof(HoleView) == 144

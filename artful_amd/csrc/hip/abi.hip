@@ -137,8 +137,9 @@ int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const blo
 }
 
 static bool batchFits(const ArtfHipBatchArgs* a) {
-    if (!a || !a->views || !a->firstTile || !a->map || !a->queue || a->viewCount < 1 || a->tiles < 1 || a->tiles > (1u << 25)) return false;
-    if ((a->viewDone == NULL) != (a->stamps == NULL)) return false;
+    if (!a || !a->views || !a->firstTile || !a->map || !a->rings || !a->queue || a->viewCount < 1 || a->tiles < 1 || a->tiles > (1u << 25))
+        return false;
+    if (a->viewCount > 65536) return false;
     const ArtfHipSchedule& s = a->schedule;
     if (s.workgroups < 1 || s.workgroups > 65536 || s.chunkTiles < 1 || s.chunkTiles > 64 || s.refill < 1 || s.refill > 64 || s.stepsPerCheck < 1 ||
         s.stepsPerCheck > 16)

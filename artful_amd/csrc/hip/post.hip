@@ -170,20 +170,18 @@ hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* con
 
 /* A batch's workspace to zero and its start stamped, by a kernel rather than a memset, so that a graph
  * that captures it clears the workspace on every replay. */
-__global__ __launch_bounds__(256) void batchResetKernel(uint32_t* queue, unsigned long long* counters, uint32_t* viewDone,
-                                                        unsigned long long* stamps, int views) {
+__global__ __launch_bounds__(256) void batchResetKernel(uint32_t* queue, unsigned long long* counters, unsigned long long* stamps, int stampCount) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i == 0) *queue = 0u;
+    if (i < ARTF_HIP_QUEUES) queue[i * ARTF_HIP_QUEUE_STRIDE] = 0u;
     if (counters && i < 2 * ARTF_HIP_COUNTER_SLOTS) counters[i] = 0ull;
-    if (viewDone && i < views) viewDone[i] = 0u;
-    if (stamps && i < views) stamps[1 + i] = 0ull;
+    if (stamps && i > 0 && i < stampCount) stamps[i] = 0ull;
     if (stamps && i == 0) stamps[0] = wall_clock64();
 }
 
 hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream) {
-    const int n = max(2 * ARTF_HIP_COUNTER_SLOTS, a.viewCount);
-    hipLaunchKernelGGL(batchResetKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a.queue, a.counters, a.viewDone, a.stamps,
-                       a.viewCount);
+    const int stamps = 1 + a.viewCount * ARTF_HIP_STAMP_SLOTS;
+    const int n = max(2 * ARTF_HIP_COUNTER_SLOTS, stamps);
+    hipLaunchKernelGGL(batchResetKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a.queue, a.counters, a.stamps, stamps);
     return hipGetLastError();
 }
 

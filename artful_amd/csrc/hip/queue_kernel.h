@@ -1,19 +1,27 @@
 /* artful-hip — the persistent, work-queued tracer for a batch of views.
  *
  * A fixed grid of workgroups stays on the device until every ray of every view has ended. Each wave
- * (64 lanes, one ray a lane) takes tiles of 8 x 8 rays from one queue (a single counter, one atomic
- * add per chunk of tiles), keeps them in a wave-private pool, and refills: once at least
+ * (64 lanes, one ray a lane) takes chunks of 8 x 8-ray tiles from a queue (a counter, one atomic add
+ * per chunk), keeps them in a wave-private pool, and refills: once at least
  * schedule.refill of its lanes are idle, it finishes their rays (the sky's light and the write, all
  * those lanes together), then hands the idle lanes the pool's next rays, in lane order (a ballot and
- * a prefix count over the idle lanes), and sets them up. Long rays near the photon sphere then hold
- * one lane, not a whole wave: the stepping loop stays full while rays of 50 and of 5000 steps mix.
+ * a prefix count over the idle lanes), and sets them up.
+ *
+ * A wave holds the rays of one view at a time: its view is a wave-uniform value, and on each change
+ * of view the wave copies the view's entry (frames, ray law, outputs: 776 bytes) into its own slot of
+ * LDS, from where every lane reads it by broadcast. The per-lane state is then only the ray's own,
+ * and a disc crossing or a ray's setup waits on LDS, not on a trip to L2, for the view's constants
+ * (which the tile kernel has in SGPRs from its arguments). When the pool's next rays belong to
+ * another view, the wave takes them once its lanes have all ended.
+ *
+ * There is a queue per XCD, each holding a contiguous eighth of the tiles: a wave takes from its own
+ * XCD's queue first, so that the eight L2s each see one compact part of the views (and the disc tables
+ * it reads), and the atomics are spread over eight counters; once its queue is empty it takes from the
+ * others in turn, which balances the end.
  *
  * Per ray, the arithmetic is the tile kernel's (tracer.h: setupRay, nystromStep, the crossings,
  * skyLight, in the same order), so each view's image equals what artfHipTrace draws for its frame,
- * bit for bit in FP32 (trace_exact.hip). What a view differs in (its frame, disc tables, ray law,
- * output) is read from its ArtfHipView in global memory, per lane, where the tile kernel has it in
- * scalar registers: lanes of one wave may hold rays of different views. Not in the reference, which
- * draws one view a frame. */
+ * bit for bit in FP32 (trace_exact.hip). Not in the reference, which draws one view a frame. */
 #pragma once
 
 #include "trace_kernel.h"
@@ -23,28 +31,7 @@ namespace artf {
 constexpr int kQueueGroup = 256;
 constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 
-/* The view, sub-ray and pixel of ray k of the batch: tile k / 64 of the queue, place k % 64 in its
- * 8 x 8 block. False for a place beyond the view's right or bottom edge. */
-struct RayPlace {
-    int view, sub, px, py;
-};
-
-__device__ __forceinline__ bool placeOf(const ArtfHipBatchArgs& a, uint32_t ray, RayPlace& p) {
-    const uint32_t tile = ray >> 6;
-    int low = 0, high = a.viewCount; /* firstTile[low] <= tile < firstTile[high] */
-    while (high - low > 1) {
-        const int mid = (low + high) >> 1;
-        if (a.firstTile[mid] <= tile) low = mid;
-        else high = mid;
-    }
-    const ArtfHipView& v = a.views[low];
-    const uint32_t local = tile - a.firstTile[low], perSub = v.tilesX * v.tilesY;
-    const uint32_t sub = local / perSub, rest = local - sub * perSub;
-    const uint32_t ty = rest / v.tilesX, tx = rest - ty * v.tilesX;
-    p.view = low, p.sub = (int)sub;
-    p.px = (int)(tx * 8u + (ray & 7u)), p.py = (int)(ty * 8u + ((ray >> 3) & 7u));
-    return p.px < (int)v.frame.size[0] && p.py < (int)v.frame.size[1];
-}
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
 
 template <typename R> __device__ __forceinline__ void eyeOfView(const ArtfHipView& v, Eye<R>& e);
 template <> __device__ __forceinline__ void eyeOfView<float>(const ArtfHipView& v, Eye<float>& e) { eyeFrom<float>(v.frame, e); }
@@ -61,10 +48,29 @@ __device__ __forceinline__ void discOfView(const ArtfHipView& v, float time, Dis
     discOf<double>(v.frame, g.disc[0], g.disc[1], g.disc[3], g.spin[2], time, d);
 }
 
-__device__ __forceinline__ DiscTables tablesOfView(const ArtfHipView& v, const float* map) {
-    DiscTables t;
-    t.map = map, t.rings = v.rings, t.glow = v.glow, t.fluid = v.fluid;
-    return t;
+/* The wave's view: its entry, tables and ray law (all wave-uniform). */
+struct WaveView {
+    const ArtfHipView* v;
+    DiscTables tables;
+    float uEscape;
+};
+
+/* The wave's copy of views[view] in its LDS slot, then the view's table addresses and ray law. */
+__device__ __forceinline__ WaveView waveViewOf(const ArtfHipView* __restrict__ views, const ArtfHipBatchArgs& a, uint32_t view,
+                                               ArtfHipView* slot, int lane) {
+    static_assert(sizeof(ArtfHipView) % 8 == 0, "the view is copied in 8-byte words");
+    const uint2* from = reinterpret_cast<const uint2*>(views + view);
+    uint2* to = reinterpret_cast<uint2*>(slot);
+    for (int i = lane; i < (int)(sizeof(ArtfHipView) / 8); i += 64) to[i] = from[i];
+    __builtin_amdgcn_wave_barrier();
+    WaveView w;
+    w.v = slot;
+    w.tables.map = a.map;
+    w.tables.rings = a.rings + (size_t)view * ARTF_HIP_RING_FLOATS;
+    w.tables.glow = w.tables.rings + ARTF_HOLE_MAP_HEIGHT * 2;
+    w.tables.fluid = a.fluids ? a.fluids + (size_t)view * ((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT) : a.map;
+    w.uEscape = w.v->escapeRadius > 0.f ? 1.f / w.v->escapeRadius : 0.f;
+    return w;
 }
 
 /* Where a ray left for the sky: the direction and drag angle at the escape. */
@@ -78,30 +84,28 @@ template <typename R> struct PlainLane {
     RayConst<R> c;
     RayState<R> s;
     Gathered<R> acc;
-    R uHorizon, uEscape;
     Away<R> away;
 
-    __device__ __forceinline__ void begin(const ArtfHipView& v, R nx, R ny) {
+    __device__ __forceinline__ void begin(const WaveView& w, R nx, R ny) {
         Eye<R> e;
-        eyeOfView<R>(v, e);
+        eyeOfView<R>(*w.v, e);
         setupRay(e, nx, ny, c, s);
         acc.r = acc.g = acc.b = (R)0, acc.through = (R)1;
-        uHorizon = e.uHorizon;
-        uEscape = (R)(v.escapeRadius > 0.f ? 1.f / v.escapeRadius : 0.f);
     }
 
     /* One step of traceRay's loop; true when the ray ends in it (sky: by escaping). */
-    __device__ __forceinline__ bool step(const ArtfHipView* views, int view, const float* map, float time, bool& sky) {
+    __device__ __forceinline__ bool step(const WaveView& w, float time, bool& sky) {
+        const ArtfHipView& v = *w.v;
         RayState<R> n;
         StepLooks<R> looks;
         nystromStep(c, s, n, looks);
         if (s.n2 * n.n2 < (R)0) {
-            const ArtfHipView& v = views[view];
             DiscConst<R> d;
             discOfView(v, time, d);
-            if (v.disc == DISC_PAINTED) crossPainted(d, c, s, n, looks, tablesOfView(v, map), acc);
-            else crossAnalytic(d, c, s, n, looks, v.emissivity, map, acc);
+            if (v.disc == DISC_PAINTED) crossPainted(d, c, s, n, looks, w.tables, acc);
+            else crossAnalytic(d, c, s, n, looks, v.emissivity, w.tables.map, acc);
         }
+        const R uEscape = (R)w.uEscape;
         const bool escaped = uEscape > n.u;
         if (escaped) {
             const R part = quot(s.u - uEscape, s.u - n.u);
@@ -111,17 +115,23 @@ template <typename R> struct PlainLane {
             away.psi = s.psi + part * (n.psi - s.psi);
             sky = true;
         }
-        if (escaped || n.u > uHorizon || !(acc.through > (R).004f)) return true;
+        if (escaped || n.u > uHorizonOf(v) || !(acc.through > (R).004f)) return true;
         s = n;
         return false;
     }
 
-    __device__ __forceinline__ void finish(const ArtfHipView& v, const float* map, bool sky, float rgb[3]) {
-        const float stars = v.frame.look[2];
-        if (stars > 0.f && sky) skyLight(stars, c, away.x, away.y, away.z, away.psi, (R)v.frame.look[1], map, acc);
+    __device__ __forceinline__ R uHorizonOf(const ArtfHipView& v) const;
+
+    __device__ __forceinline__ void finish(const WaveView& w, bool sky, float rgb[3]) {
+        const float stars = w.v->frame.look[2];
+        if (stars > 0.f && sky) skyLight(stars, c, away.x, away.y, away.z, away.psi, (R)w.v->frame.look[1], w.tables.map, acc);
         rgb[0] = (float)acc.r, rgb[1] = (float)acc.g, rgb[2] = (float)acc.b;
     }
 };
+
+/* The capture radius as eyeFrom / eyeFrom64 give it. */
+template <> __device__ __forceinline__ float PlainLane<float>::uHorizonOf(const ArtfHipView& v) const { return v.frame.spin[1]; }
+template <> __device__ __forceinline__ double PlainLane<double>::uHorizonOf(const ArtfHipView& v) const { return v.frame64.spin[1]; }
 
 /* A lane's ray in the mixed tracer: traceRayMixed's state (FP32 steps, FP64 where they are
  * ill-conditioned). */
@@ -131,24 +141,22 @@ struct MixedLane {
     RayConst<float> cf;
     RayState<float> s;
     Gathered<double> acc;
-    double uHorizon, uEscape;
-    float uNear, uEscapeF;
     Away<double> away;
 
-    __device__ __forceinline__ void begin(const ArtfHipView& v, double nx, double ny) {
+    __device__ __forceinline__ void begin(const WaveView& w, double nx, double ny) {
         Eye<double> e;
-        eyeFrom64(v.frame64, e);
+        eyeFrom64(w.v->frame64, e);
         RayState<double> sd;
         setupRay(e, nx, ny, cd, sd);
         cf = narrowConst(cd);
         s = castState<float>(sd);
         acc.r = acc.g = acc.b = 0.0, acc.through = 1.0;
-        const float escape = v.escapeRadius > 0.f ? 1.f / v.escapeRadius : 0.f;
-        uHorizon = e.uHorizon, uEscape = (double)escape;
-        uNear = (float)(.9 * uHorizon), uEscapeF = escape;
     }
 
-    __device__ __forceinline__ bool step(const ArtfHipView* views, int view, const float* map, float time, bool& sky) {
+    __device__ __forceinline__ bool step(const WaveView& w, float time, bool& sky) {
+        const ArtfHipView& v = *w.v;
+        const double uHorizon = v.frame64.spin[1], uEscape = (double)w.uEscape;
+        const float uNear = (float)(.9 * uHorizon), uEscapeF = w.uEscape;
         RayState<float> n;
         StepLooks<float> looks;
         nystromStep(cf, s, n, looks);
@@ -157,24 +165,23 @@ struct MixedLane {
             s = n;
             return false;
         }
-        const RayState<double> w = castState<double>(s);
+        const RayState<double> x = castState<double>(s);
         RayState<double> m;
         StepLooks<double> l;
-        nystromStep(cd, w, m, l);
-        if (w.n2 * m.n2 < 0.0) {
-            const ArtfHipView& v = views[view];
+        nystromStep(cd, x, m, l);
+        if (x.n2 * m.n2 < 0.0) {
             DiscConst<double> d;
             discOfView(v, time, d);
-            if (v.disc == DISC_PAINTED) crossPainted(d, cd, w, m, l, tablesOfView(v, map), acc);
-            else crossAnalytic(d, cd, w, m, l, v.emissivity, map, acc);
+            if (v.disc == DISC_PAINTED) crossPainted(d, cd, x, m, l, w.tables, acc);
+            else crossAnalytic(d, cd, x, m, l, v.emissivity, w.tables.map, acc);
         }
         const bool escaped = uEscape > m.u;
         if (escaped) {
-            const double part = quot(w.u - uEscape, w.u - m.u);
-            away.x = w.n0 + part * (m.n0 - w.n0);
-            away.y = w.n1 + part * (m.n1 - w.n1);
-            away.z = w.n2 + part * (m.n2 - w.n2);
-            away.psi = w.psi + part * (m.psi - w.psi);
+            const double part = quot(x.u - uEscape, x.u - m.u);
+            away.x = x.n0 + part * (m.n0 - x.n0);
+            away.y = x.n1 + part * (m.n1 - x.n1);
+            away.z = x.n2 + part * (m.n2 - x.n2);
+            away.psi = x.psi + part * (m.psi - x.psi);
             sky = true;
         }
         if (escaped || m.u > uHorizon || !(acc.through > .004)) return true;
@@ -182,9 +189,9 @@ struct MixedLane {
         return false;
     }
 
-    __device__ __forceinline__ void finish(const ArtfHipView& v, const float* map, bool sky, float rgb[3]) {
-        const float stars = v.frame.look[2];
-        if (stars > 0.f && sky) skyLight(stars, cd, away.x, away.y, away.z, away.psi, (double)v.frame.look[1], map, acc);
+    __device__ __forceinline__ void finish(const WaveView& w, bool sky, float rgb[3]) {
+        const float stars = w.v->frame.look[2];
+        if (stars > 0.f && sky) skyLight(stars, cd, away.x, away.y, away.z, away.psi, (double)w.v->frame.look[1], w.tables.map, acc);
         rgb[0] = (float)acc.r, rgb[1] = (float)acc.g, rgb[2] = (float)acc.b;
     }
 };
@@ -193,87 +200,135 @@ template <int Precision> struct LaneOf { using type = PlainLane<float>; };
 template <> struct LaneOf<ARTF_PRECISION_FP64> { using type = PlainLane<double>; };
 template <> struct LaneOf<ARTF_PRECISION_MIXED> { using type = MixedLane; };
 
+/* The first tile of queue q's share: q * tiles / ARTF_HIP_QUEUES. */
+__device__ __forceinline__ uint32_t segmentEdge(uint32_t tiles, uint32_t q) { return (uint32_t)(((uint64_t)tiles * q) / ARTF_HIP_QUEUES); }
+
+/* The view holding tile t: the last v with firstTile[v] <= t (a wave-uniform search). */
+__device__ __forceinline__ uint32_t viewOfTile(const uint32_t* __restrict__ firstTile, int viewCount, uint32_t tile) {
+    uint32_t low = 0, high = (uint32_t)viewCount;
+    while (high - low > 1) {
+        const uint32_t mid = (low + high) >> 1;
+        if (firstTile[mid] <= tile) low = mid;
+        else high = mid;
+    }
+    return uniform(low);
+}
+
+/* When the wave leaves a view (or ends): the last moment its rays of that view ended, into one of the
+ * view's slots, the later of the two kept. One atomic per wave and view, its result not waited for. */
+__device__ __forceinline__ void stampView(const ArtfHipBatchArgs& a, uint32_t view, unsigned long long& lastEnd, int lane) {
+    if (!a.stamps || !lastEnd) return;
+    if (lane == 0) {
+        const unsigned slot = (blockIdx.x * (kQueueGroup / 64) + (threadIdx.x >> 6)) % ARTF_HIP_STAMP_SLOTS;
+        atomicMax(&a.stamps[1 + view * ARTF_HIP_STAMP_SLOTS + slot], lastEnd);
+    }
+    lastEnd = 0;
+}
+
+/* views and firstTile come apart from the rest as restrict arguments: nothing the kernel writes can
+ * alias them, so their wave-uniform reads are scalar loads. */
 template <int Precision>
-__global__ __launch_bounds__(kQueueGroup) void queueKernel(ArtfHipBatchArgs a) {
+__global__ __launch_bounds__(kQueueGroup) void queueKernel(const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile,
+                                                           ArtfHipBatchArgs a) {
     using Lane = typename LaneOf<Precision>::type;
     const int lane = threadIdx.x & 63;
     const unsigned long long below = (1ull << lane) - 1ull; /* the lanes before this one */
     const int refillAt = a.schedule.refill, chunk = a.schedule.chunkTiles, stride = a.schedule.stepsPerCheck;
 
+    __shared__ ArtfHipView cached[kQueueGroup / 64];
+    ArtfHipView* slot = &cached[threadIdx.x >> 6];
     Lane ray;
     bool active = false, pending = false, sky = false;
-    int steps = 0, maxSteps = 0, view = 0;
+    int steps = 0;
     uint32_t outAt = 0;
     float time = 0.f;
     unsigned long long stepSum = 0, slots = 0;
-    uint32_t poolNext = 0, poolEnd = 0; /* the wave's rays not yet given out (the same in every lane) */
+    /* wave-uniform: the pool of rays taken from the queue and not yet given out, [poolNext, poolEnd);
+     * the wave's view, whose rays are [viewBegin, viewEnd) */
+    uint32_t poolNext = 0, poolEnd = 0, view = 0, viewBegin = 0, viewEnd = 0;
+    WaveView w = waveViewOf(views, a, 0, slot, lane);
     bool drained = false;
+    const uint32_t xcd = blockIdx.x % ARTF_HIP_QUEUES;
+    uint32_t empty = 0; /* the queues this wave has found empty (a bit each) */
+    unsigned long long lastEnd = 0; /* when rays of the wave's view last ended (wave-uniform) */
 
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         const int idleCount = __popcll(idle);
         if (idleCount >= refillAt) {
             /* finish the rays that ended since the last refill, all together */
-            if (pending) {
-                const ArtfHipView& v = a.views[view];
-                float rgb[3];
-                ray.finish(v, a.map, sky, rgb);
-                v.out[0][outAt] = rgb[0];
-                v.out[1][outAt] = rgb[1];
-                v.out[2][outAt] = rgb[2];
-            }
-            if (a.viewDone) {
-                /* each view's count of rays done, one add per view among the finished lanes; the
-                 * add that completes a view stamps the moment */
-                unsigned long long left = __ballot(pending);
-                while (left) {
-                    const int first = __ffsll((long long)left) - 1;
-                    const int v0 = __shfl(view, first);
-                    const unsigned long long same = __ballot(pending && view == v0);
-                    if (lane == first) {
-                        const uint32_t count = (uint32_t)__popcll(same);
-                        if (atomicAdd(&a.viewDone[v0], count) + count == a.views[v0].rays) a.stamps[1 + v0] = wall_clock64();
-                    }
-                    left &= ~same;
+            const unsigned long long ended = __ballot(pending);
+            if (ended) {
+                if (pending) {
+                    float rgb[3];
+                    ray.finish(w, sky, rgb);
+                    const ArtfHipView& v = *w.v;
+                    v.out[0][outAt] = rgb[0];
+                    v.out[1][outAt] = rgb[1];
+                    v.out[2][outAt] = rgb[2];
                 }
+                if (a.stamps) lastEnd = wall_clock64();
+                pending = false;
             }
-            pending = false;
-            /* the pool's next rays to the idle lanes, in lane order */
-            if (!drained) {
-                const int rank = __popcll(idle & below);
-                uint32_t mine = kNoRay;
-                int given = 0;
-                while (given < idleCount) {
-                    if (poolNext == poolEnd) {
+            /* the pool's next rays of the wave's view to the idle lanes, in lane order */
+            const int rank = __popcll(idle & below);
+            uint32_t mine = kNoRay;
+            int given = 0;
+            while (given < idleCount && !drained) {
+                if (poolNext == poolEnd) {
+                    /* a chunk from this XCD's queue, else from the next that has tiles left */
+                    bool got = false;
+                    for (int t = 0; t < ARTF_HIP_QUEUES && !got; ++t) {
+                        const uint32_t q = (xcd + (uint32_t)t) % ARTF_HIP_QUEUES;
+                        if (empty & (1u << q)) continue;
+                        const uint32_t begin = segmentEdge(a.tiles, q), end = segmentEdge(a.tiles, q + 1);
                         uint32_t base = 0;
-                        if (lane == 0) base = atomicAdd(a.queue, (uint32_t)chunk);
-                        base = __shfl(base, 0);
-                        if (base >= a.tiles) {
-                            drained = true;
-                            break;
+                        if (lane == 0) base = atomicAdd(&a.queue[q * ARTF_HIP_QUEUE_STRIDE], (uint32_t)chunk);
+                        base = uniform(base);
+                        if (base >= end - begin) {
+                            empty |= 1u << q;
+                            continue;
                         }
-                        poolNext = base * 64u;
-                        poolEnd = min(base + (uint32_t)chunk, a.tiles) * 64u;
+                        poolNext = (begin + base) * 64u;
+                        poolEnd = min(begin + base + (uint32_t)chunk, end) * 64u;
+                        got = true;
                     }
-                    const int take = min(idleCount - given, (int)(poolEnd - poolNext));
-                    if (!active && rank >= given && rank < given + take) mine = poolNext + (uint32_t)(rank - given);
-                    poolNext += (uint32_t)take;
-                    given += take;
+                    if (!got) {
+                        drained = true;
+                        break;
+                    }
                 }
-                RayPlace p;
-                if (mine != kNoRay && placeOf(a, mine, p)) {
-                    const ArtfHipView& v = a.views[p.view];
-                    const ArtfHoleFrame& f = v.frame;
-                    const int side = (int)f.size[2];
+                if (poolNext >= viewEnd || poolNext < viewBegin) {
+                    /* another view's rays: taken once no lane holds a ray of this one */
+                    if (given > 0 || __ballot(active)) break;
+                    stampView(a, view, lastEnd, lane);
+                    view = viewOfTile(firstTile, a.viewCount, poolNext >> 6);
+                    viewBegin = firstTile[view] * 64u;
+                    viewEnd = firstTile[view + 1] * 64u;
+                    w = waveViewOf(views, a, view, slot, lane);
+                }
+                const int take = (int)min((uint32_t)(idleCount - given), min(poolEnd, viewEnd) - poolNext);
+                if (!active && rank >= given && rank < given + take) mine = poolNext + (uint32_t)(rank - given);
+                poolNext += (uint32_t)take;
+                given += take;
+            }
+            if (mine != kNoRay) {
+                const ArtfHipView& v = *w.v;
+                const ArtfHoleFrame& f = v.frame;
+                const uint32_t local = (mine >> 6) - (viewBegin >> 6), perSub = v.tilesX * v.tilesY;
+                const uint32_t sub = local / perSub, rest = local - sub * perSub;
+                const uint32_t ty = rest / v.tilesX, tx = rest - ty * v.tilesX;
+                const int px = (int)(tx * 8u + (mine & 7u)), py = (int)(ty * 8u + ((mine >> 3) & 7u));
+                const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+                if (px < width && py < height) { /* (a tile's places beyond the view's edges stay idle) */
                     float dx, dy;
-                    subPixel(v.jitter, p.px, p.py, p.sub % side, p.sub / side, side, dx, dy);
-                    time = rayMoment(f, p.sub, side);
+                    subPixel(v.jitter, px, py, (int)sub % side, (int)sub / side, side, dx, dy);
+                    time = rayMoment(f, (int)sub, side);
                     typename Lane::Aim nx, ny;
-                    aimOf(f, p.px, p.py, dx, dy, nx, ny);
-                    ray.begin(v, nx, ny);
-                    const uint32_t width = f.size[0], height = f.size[1];
-                    outAt = (side == 1 ? 0u : (uint32_t)p.sub * width * height) + (uint32_t)p.py * width + (uint32_t)p.px;
-                    view = p.view, maxSteps = v.maxSteps, steps = 0, sky = false, active = true;
+                    aimOf(f, px, py, dx, dy, nx, ny);
+                    ray.begin(w, nx, ny);
+                    outAt = (side == 1 ? 0u : sub * (uint32_t)width * (uint32_t)height) + (uint32_t)py * (uint32_t)width + (uint32_t)px;
+                    steps = 0, sky = false, active = true;
                 }
             }
             if (!__ballot(active)) {
@@ -281,13 +336,14 @@ __global__ __launch_bounds__(kQueueGroup) void queueKernel(ArtfHipBatchArgs a) {
                 continue;
             }
         }
+        const int maxSteps = w.v->maxSteps;
         for (int k = 0; k < stride; ++k) {
             if (!__ballot(active)) break;
             slots += 64;
             if (active) {
-                const bool ended = ray.step(a.views, view, a.map, time, sky);
+                const bool over = ray.step(w, time, sky);
                 ++steps;
-                if (ended || steps >= maxSteps) {
+                if (over || steps >= maxSteps) {
                     active = false, pending = true;
                     stepSum += (unsigned long long)steps;
                 }
@@ -295,6 +351,7 @@ __global__ __launch_bounds__(kQueueGroup) void queueKernel(ArtfHipBatchArgs a) {
         }
     }
 
+    stampView(a, view, lastEnd, lane);
     if (a.counters) {
         for (int offset = 32; offset > 0; offset >>= 1) stepSum += __shfl_xor(stepSum, offset);
         if (lane == 0) {
@@ -306,7 +363,7 @@ __global__ __launch_bounds__(kQueueGroup) void queueKernel(ArtfHipBatchArgs a) {
 }
 
 template <int Precision> inline hipError_t launchQueueWith(const ArtfHipBatchArgs& a, hipStream_t stream) {
-    hipLaunchKernelGGL(queueKernel<Precision>, dim3((unsigned)a.schedule.workgroups), dim3(kQueueGroup), 0, stream, a);
+    hipLaunchKernelGGL(queueKernel<Precision>, dim3((unsigned)a.schedule.workgroups), dim3(kQueueGroup), 0, stream, a.views, a.firstTile, a);
     return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@
 #include "artful/batch.h"
 #include "artful_hip.h"
 
-#define RING_FLOATS (ARTF_HOLE_MAP_HEIGHT * 3) /* rows (speed, heat) then glow, per view */
+#define RING_FLOATS ARTF_HIP_RING_FLOATS
 #define FLUID_FLOATS ((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT)
 
 typedef struct BatchView {
@@ -33,7 +33,7 @@ struct ArtfHoleBatch {
     float* fluids;            /* device, count * FLUID_FLOATS, painted disc only */
     ArtfHipView* deviceViews; /* device, count */
     uint32_t* firstTile;      /* device, count + 1 */
-    void* workspace;          /* device: queue, counters, viewDone, stamps */
+    void* workspace;          /* device: queue, counters, stamps */
     ArtfHipBatchArgs args;
     unsigned long long* countersHost;
     unsigned long long* stampsHost;
@@ -76,8 +76,8 @@ static ArtfHoleSchedule resolved(const ArtfHoleBatch* b, const ArtfHoleSchedule*
     if (r.workgroups <= 0) r.workgroups = b->computeUnits * occupancy;
     r.workgroups = artfClampi(r.workgroups, 1, 65536);
     r.chunkTiles = r.chunkTiles <= 0 ? 2 : artfClampi(r.chunkTiles, 1, 64);
-    r.refill = r.refill <= 0 ? 32 : artfClampi(r.refill, 1, 64);
-    r.stepsPerCheck = r.stepsPerCheck <= 0 ? 1 : artfClampi(r.stepsPerCheck, 1, 16);
+    r.refill = r.refill <= 0 ? 64 : artfClampi(r.refill, 1, 64);
+    r.stepsPerCheck = r.stepsPerCheck <= 0 ? 8 : artfClampi(r.stepsPerCheck, 1, 16);
     return r;
 }
 
@@ -167,9 +167,6 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
         ArtfHipView* h = &hv[v];
         h->frame = bv->frame;
         artfHoleFrame64Of(&d, &given->camera, bv->width, bv->height, &h->frame64);
-        h->rings = b->rings + (size_t)v * RING_FLOATS;
-        h->glow = h->rings + ARTF_HOLE_MAP_HEIGHT * 2;
-        h->fluid = painted ? b->fluids + (size_t)v * FLUID_FLOATS : b->map;
         for (int c = 0; c < 3; ++c) h->out[c] = bv->side > 1 ? bv->rays[c] : bv->hdr[c];
         h->maxSteps = t.maxSteps, h->escapeRadius = t.escapeRadius, h->disc = t.disc, h->emissivity = t.emissivity, h->jitter = t.jitter;
         h->tilesX = (uint32_t)(bv->width + 7) / 8, h->tilesY = (uint32_t)(bv->height + 7) / 8;
@@ -180,9 +177,9 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     }
     firstTile[count] = at;
 
-    const size_t slots = 2 * ARTF_HIP_COUNTER_SLOTS;
-    const size_t doneWords = ((size_t)count + 1) & ~(size_t)1; /* uint32 words, to an 8-byte edge */
-    const size_t workBytes = 64 + slots * 8 + doneWords * 4 + ((size_t)count + 1) * 8;
+    const size_t slots = 2 * ARTF_HIP_COUNTER_SLOTS, stamps = 1 + (size_t)count * ARTF_HIP_STAMP_SLOTS;
+    const size_t queueBytes = (size_t)ARTF_HIP_QUEUES * ARTF_HIP_QUEUE_STRIDE * 4;
+    const size_t workBytes = queueBytes + slots * 8 + stamps * 8;
     if (ready) {
         b->deviceViews = artfHipMalloc((size_t)count * sizeof(ArtfHipView));
         b->firstTile = artfHipMalloc((size_t)(count + 1) * sizeof(uint32_t));
@@ -198,13 +195,12 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     char* w = (char*)b->workspace;
     ArtfHipBatchArgs* a = &b->args;
     a->views = b->deviceViews, a->firstTile = b->firstTile, a->viewCount = count, a->tiles = at;
-    a->precision = t.precision, a->map = b->map;
+    a->precision = t.precision, a->map = b->map, a->rings = b->rings, a->fluids = b->fluids;
     a->queue = (uint32_t*)w;
-    a->counters = (unsigned long long*)(w + 64);
-    a->viewDone = (uint32_t*)(w + 64 + slots * 8);
-    a->stamps = (unsigned long long*)(w + 64 + slots * 8 + doneWords * 4);
+    a->counters = (unsigned long long*)(w + queueBytes);
+    a->stamps = (unsigned long long*)(w + queueBytes + slots * 8);
     b->countersHost = artfMalloc(slots * 8);
-    b->stampsHost = artfCalloc((size_t)count + 1, 8);
+    b->stampsHost = artfCalloc(stamps, 8);
     b->schedule = resolved(b, NULL);
     return b;
 }
@@ -240,7 +236,7 @@ bool artfHoleBatchDraw(ArtfHoleBatch* b) {
     }
     b->batchMs = artfHipEventMs(b->events[0], b->events[1]);
     if (artfHipDownload(b->countersHost, b->args.counters, 2 * ARTF_HIP_COUNTER_SLOTS * 8, b->stream) ||
-        artfHipDownload(b->stampsHost, b->args.stamps, ((size_t)b->count + 1) * 8, b->stream))
+        artfHipDownload(b->stampsHost, b->args.stamps, (1 + (size_t)b->count * ARTF_HIP_STAMP_SLOTS) * 8, b->stream))
         return false;
     b->drawn = true;
     return true;
@@ -256,7 +252,12 @@ bool artfHoleBatchReadHdr(ArtfHoleBatch* b, int view, float* hdr) {
 }
 
 static double stampMs(const ArtfHoleBatch* b, int view) {
-    const unsigned long long start = b->stampsHost[0], end = b->stampsHost[1 + view];
+    const unsigned long long start = b->stampsHost[0];
+    unsigned long long end = 0;
+    for (int k = 0; k < ARTF_HIP_STAMP_SLOTS; ++k) {
+        const unsigned long long t = b->stampsHost[1 + (size_t)view * ARTF_HIP_STAMP_SLOTS + k];
+        end = t > end ? t : end;
+    }
     if (b->clockHz <= 0 || end < start) return -1.0;
     return (double)(end - start) * 1e3 / b->clockHz;
 }

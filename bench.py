@@ -145,6 +145,139 @@ def adjoint_fit(native, iterations=200):
                     "hidden": list(K.HIDDEN), "start": list(K.START)}}
 
 
+def per_view_tiles(native, views, trc, reps=3):
+    """The same views traced by the tile kernel (artfHipTrace), one launch per view back to back on one
+    stream: GPU ms of the whole sequence (best of reps), its steps and lane-slots. Analytic disc only
+    (its tables are the map's shared colours and sky, so one map serves every view)."""
+    import ctypes as C
+    from artful_amd import abi
+    L = native.lib()
+    themap = native.Scene(*views[0][:2]).map()
+    dmap = L.artfHipMalloc(themap.nbytes)
+    stream = L.artfHipStreamCreate()
+    L.artfHipUpload(dmap, themap.ctypes.data, themap.nbytes, stream)
+    counters = L.artfHipMalloc(2 * 1024 * 8)
+    args, planes = [], []
+    for desc, cam, w, h in views:
+        f, f64 = native.frames_for(desc, cam, w, h)
+        a = abi.HipTraceArgs()
+        a.frame, a.frame64, a.map, a.fluid, a.counters = f, f64, dmap, dmap, counters
+        for c in range(3):
+            buf = L.artfHipMalloc(w * h * desc.samples ** 2 * 4)
+            planes.append(buf)
+            a.out[c] = buf
+        a.precision, a.maxSteps, a.escapeRadius, a.disc, a.emissivity = trc.precision, trc.maxSteps, trc.escapeRadius, trc.disc, trc.emissivity
+        args.append(a)
+    e0, e1 = L.artfHipEventCreate(), L.artfHipEventCreate()
+    best, steps, slots = 1e30, 0, 0
+    host = np.zeros(2 * 1024, np.uint64)
+    for _ in range(reps):
+        L.artfHipMemset(counters, 0, 2 * 1024 * 8, stream)
+        L.artfHipEventRecord(e0, stream)
+        for a in args:
+            if L.artfHipTrace(C.byref(a), stream):
+                raise RuntimeError("artfHipTrace failed")
+        L.artfHipEventRecord(e1, stream)
+        L.artfHipStreamSync(stream)
+        best = min(best, L.artfHipEventMs(e0, e1))
+        L.artfHipDownload(host.ctypes.data, counters, host.nbytes, stream)
+        steps, slots = int(host[:1024].sum()), int(host[1024:].sum())
+    for b in planes + [dmap, counters]:
+        L.artfHipFree(b)
+    L.artfHipEventDestroy(e0), L.artfHipEventDestroy(e1), L.artfHipStreamDestroy(stream)
+    return {"ms": best, "steps": steps, "lane_slots": slots}
+
+
+def batched_views(native, count=64, seed=4, max_steps=5000, draws=5):
+    """64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
+    U[5,85], eye r U[20,100], disc outer U[10,40], each with its own rings and fluid; rays of up to 5000
+    steps; one persistent launch replayed from a HIP graph, schedule from the tuner. Beside it, the
+    analytic disc through the same batch and through the tile kernel one launch per view."""
+    from artful_amd.workloads import uneven_views
+    views, _ = uneven_views(count, seed)
+    out = {"views": count, "seed": seed, "max_steps": max_steps}
+    for disc in ("painted", "analytic"):
+        trc = native.tracing(precision="fp32", disc=disc, max_steps=max_steps)
+        t0 = time.perf_counter()
+        b = native.HoleBatch(views, trc)
+        setup = time.perf_counter() - t0
+
+        def run(n):
+            ms, trace = [], []
+            for _ in range(n):
+                b.draw()
+                st = b.stats()
+                ms.append(st.batchMs)
+                trace.append(st.traceMs)
+            return st, float(np.median(ms)), float(np.median(trace))
+
+        b.draw()
+        st0, default_ms, _ = run(draws)
+        t0 = time.perf_counter()
+        sched = b.tune()
+        tune_s = time.perf_counter() - t0
+        st, batch_ms, trace_ms = run(draws)
+        lat = b.latency()
+        b.close()
+        leg = {"setup_s": setup, "rays": int(st.rays), "steps_per_ray": st.steps / st.rays,
+               "graph_ms": batch_ms, "trace_ms": trace_ms, "rays_per_s": st.rays / (batch_ms * 1e-3),
+               "active_lane_fraction": st.steps / max(st.laneSlots, 1),
+               "default_schedule_ms": default_ms, "default_active_lane_fraction": st0.steps / max(st0.laneSlots, 1),
+               "schedule": {"workgroups": sched.workgroups, "chunk_tiles": sched.chunkTiles, "refill": sched.refill,
+                            "steps_per_check": sched.stepsPerCheck}, "tune_s": tune_s,
+               "latency_ms": {"min": float(lat.min()), "median": float(np.median(lat)), "max": float(lat.max()),
+                              "per_view": [round(float(v), 4) for v in lat]}}
+        if disc == "analytic":
+            tiles = per_view_tiles(native, views, trc)
+            leg["tile_kernel_per_view"] = {"ms": tiles["ms"], "rays_per_s": st.rays / (tiles["ms"] * 1e-3),
+                                           "active_lane_fraction": tiles["steps"] / max(tiles["lane_slots"], 1),
+                                           "launches": count}
+            leg["speedup_over_tile_kernel"] = tiles["ms"] / batch_ms
+        out[disc] = leg
+        log(f"batch {disc}: {leg}")
+    return out
+
+
+def precision_study(native, width=3840, height=2160):
+    """3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
+    75, 89}, eye at r 50, painted disc ISCO..20, Doppler 1: FP32, mixed and FP64 traces, each image's
+    relative HDR error against its FP64 trace, and rays/s."""
+    from artful_amd.workloads import view_of
+    r = native.Renderer(width, height)
+    scene = native.Scene()
+    rows = []
+    for spin in (0.0, .5, .9, .998):
+        for incl in (17, 45, 75, 89):
+            desc, cam, _, _ = view_of(spin, incl, 50.0, 20.0, width, height, fov_y=.35)
+            scene.set_hole(desc)
+            scene.set_camera(cam)
+            row = {"spin": spin, "inclination": incl}
+            hdr = {}
+            for p in ("fp64", "fp32", "mixed"):
+                r.set_tracing(p)
+                ms = []
+                for _ in range(3):
+                    if not r.draw(scene):
+                        raise RuntimeError("draw failed")
+                    ms.append(r.hole_stats().traceMs)
+                hdr[p] = r.read_hdr()
+                row[f"{p}_rays_per_s"] = width * height / (min(ms) * 1e-3)
+            ref = hdr["fp64"].astype(np.float64)
+            for p in ("fp32", "mixed"):
+                e = rel_error(hdr[p], ref)
+                rel = np.abs(hdr[p].astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-3)
+                e["p99.9"] = float(np.quantile(rel, .999))
+                row[f"{p}_error"] = e
+            row["mixed_over_fp64"] = row["mixed_rays_per_s"] / row["fp64_rays_per_s"]
+            rows.append(row)
+            log(f"precision study: {row}")
+    r.close()
+    scene.close()
+    summary = {p: {"max_of_max": max(x[f"{p}_error"]["max"] for x in rows), "max_of_mean": max(x[f"{p}_error"]["mean"] for x in rows),
+                   "max_share_over_1e-3": max(x[f"{p}_error"]["share_over_1e-3"] for x in rows)} for p in ("fp32", "mixed")}
+    return {"width": width, "height": height, "images": rows, "summary": summary}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +285,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
+    ap.add_argument("--only", choices=["batch", "study", "adjoint"], help="run one extra leg alone and print its JSON")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
     args = ap.parse_args()
@@ -171,6 +307,10 @@ def main():
                 raise RuntimeError("copy failed")
         L.artfHipFree(a), L.artfHipFree(b)
         print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 5}))
+        return
+    if args.only:
+        leg = {"batch": batched_views, "study": precision_study, "adjoint": adjoint_fit}[args.only](native)
+        print(json.dumps({args.only: leg}), flush=True)
         return
     desc, cam = workload_scene(native)
     scene = native.Scene(desc, cam)
@@ -201,6 +341,10 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(native)
     adjoint = adjoint_fit(native)
     log(f"adjoint: {adjoint}")
+    renderer.close()
+    scene.close()
+    batch = None if args.no_batch else batched_views(native)
+    study = None if args.no_study else precision_study(native)
 
     line = {
         "metric": "Kerr rays/s at 1080p (FP32 vs FP64 plain, <=1e-3 rel err); adjoint/forward time",
@@ -221,6 +365,8 @@ def main():
         "rel_error_vs_fp64": errors,
         "adjoint_over_forward": adjoint["adjoint_over_forward"],
         "adjoint": adjoint,
+        "batched_views": batch,
+        "precision_study": study,
     }
     print(json.dumps(line), flush=True)
 

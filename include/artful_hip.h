@@ -78,27 +78,29 @@ int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* cons
 
 /* ---- batches of views (not in the reference, which draws one view a frame): many views of any
  * size, spin, eye and disc, traced by one persistent launch whose waves take 8x8 tiles of rays from a
- * queue and give each lane a new ray as soon as its last one ends. Each view's rays are those
- * artfHipTrace would trace for its frame, bit for bit in FP32. ---- */
+ * queue and give idle lanes new rays of the view they are on (a wave holds one view's rays at a time).
+ * Each view's rays are those artfHipTrace would trace for its frame, bit for bit in FP32. ---- */
 
 typedef struct ArtfHipView {
     ArtfHoleFrame frame;     /* artfHoleFrame's FP32 uniforms for this view */
     ArtfHoleFrame64 frame64; /* read by the MIXED and FP64 tracers */
-    /* device tables of this view's disc: ring speed and heat (ARTF_HOLE_MAP_HEIGHT x 2, as the map's
-     * rows part at ARTF_HOLE_ROWS_AT), glow per ring (ARTF_HOLE_MAP_HEIGHT, as at ARTF_HOLE_GLOW_AT),
-     * the fluid field (ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT; read by the painted disc only) */
-    const float* rings;
-    const float* glow;
-    const float* fluid;
-    float* out[3];        /* device planes, as artfHipTrace's out[] for this view's frame */
-    int32_t maxSteps;     /* at least 1 */
-    float escapeRadius;   /* 0: where 1/r crosses 0 */
-    int32_t disc;         /* ArtfDiscModel */
-    float emissivity;     /* the analytic disc's q */
-    uint32_t jitter;      /* 0: the stratified grid; else a seed */
+    float* out[3];           /* device planes, as artfHipTrace's out[] for this view's frame */
+    int32_t maxSteps;        /* at least 1 */
+    float escapeRadius;      /* 0: where 1/r crosses 0 */
+    int32_t disc;            /* ArtfDiscModel */
+    float emissivity;        /* the analytic disc's q */
+    uint32_t jitter;         /* 0: the stratified grid; else a seed */
     uint32_t tilesX, tilesY; /* (width + 7) / 8, (height + 7) / 8 */
-    uint32_t rays;        /* width * height * side^2 */
+    uint32_t rays;           /* width * height * side^2 */
 } ArtfHipView;
+
+/* A view's ring tables in ArtfHipBatchArgs.rings: ring speed and heat (ARTF_HOLE_MAP_HEIGHT x 2, as
+ * the map's part at ARTF_HOLE_ROWS_AT), then glow per ring (ARTF_HOLE_MAP_HEIGHT, as at
+ * ARTF_HOLE_GLOW_AT). */
+#define ARTF_HIP_RING_FLOATS (ARTF_HOLE_MAP_HEIGHT * 3)
+#define ARTF_HIP_STAMP_SLOTS 16
+#define ARTF_HIP_QUEUES 8
+#define ARTF_HIP_QUEUE_STRIDE 32
 
 /* How the persistent launch spends the device. */
 typedef struct ArtfHipSchedule {
@@ -116,20 +118,27 @@ typedef struct ArtfHipBatchArgs {
     uint32_t tiles;            /* firstTile[viewCount] */
     int32_t precision;         /* ArtfPrecision, the same for every view */
     const float* map;          /* device, ARTF_HOLE_MAP_FLOATS: its gas, colours, sky and dust are shared */
-    /* device workspace, cleared by artfHipBatchReset before each launch: the queue's next tile (1 word) */
+    const float* rings;        /* device, viewCount * ARTF_HIP_RING_FLOATS: view v's at v * ARTF_HIP_RING_FLOATS */
+    /* device, viewCount * ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT: view v's fluid field at v times that; read
+     * for painted views only (NULL if there are none) */
+    const float* fluids;
+    /* device workspace, cleared by artfHipBatchReset before each launch: ARTF_HIP_QUEUES counters, each
+     * ARTF_HIP_QUEUE_STRIDE words from the last. Queue q hands out the tiles [q * tiles / ARTF_HIP_QUEUES,
+     * (q + 1) * tiles / ARTF_HIP_QUEUES) to the workgroups of its XCD (blockIdx % ARTF_HIP_QUEUES, the
+     * order workgroups are dealt to XCDs), which then take from the other queues until all are empty. */
     uint32_t* queue;
     /* device, 2 * ARTF_HIP_COUNTER_SLOTS, as artfHipTrace's counters (lane-slots: every lane of every
      * stepping wave, idle ones included); NULL: not counted */
     unsigned long long* counters;
-    /* device, viewCount words: rays of each view done so far; and stamps, viewCount + 1 words: the
-     * launch's start (stamps[0]) and the moment each view's last ray ended (stamps[1 + v]), in the
-     * device's wall clock (artfHipWallClockHz). Both NULL: not kept. */
-    uint32_t* viewDone;
+    /* device, 1 + viewCount * ARTF_HIP_STAMP_SLOTS words, or NULL: the launch's start (stamps[0]), then
+     * per view v the latest moment a wave finished rays of it, in slots 1 + v * ARTF_HIP_STAMP_SLOTS + k
+     * (the view's last ray ended at the largest of its slots), in the device's wall clock
+     * (artfHipWallClockHz). Spread over slots, as the counters are. */
     unsigned long long* stamps;
     ArtfHipSchedule schedule;
 } ArtfHipBatchArgs;
 
-/* Clears the queue, the counters, viewDone and the stamps, and stamps the start, with a kernel (no
+/* Clears the queue, the counters and the stamps, and stamps the start, with a kernel (no
  * memset node, so that the pair can be captured in a graph and replayed). */
 int artfHipBatchReset(const ArtfHipBatchArgs* args, void* stream);
 int artfHipTraceViews(const ArtfHipBatchArgs* args, void* stream);
