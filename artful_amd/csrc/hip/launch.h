@@ -12,6 +12,11 @@ namespace artf {
 constexpr int kTile = 16;
 constexpr int kGroup = 256;
 
+/* Waves per SIMD the tracers are built for: 4 for the FP32 ones (at most 128 VGPRs, which they fit
+ * in); the FP64 and mixed ones would take 250 VGPRs for 2 waves, and are held to 168 for 3, which
+ * spills a little in the rarely taken FP64 paths and measures 5% faster (4 spills too much). */
+constexpr int wavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP64 || precision == ARTF_PRECISION_MIXED ? 3 : 4; }
+
 /* FP32, the reference's arithmetic bit for bit (trace_exact.hip, contraction off). */
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
 /* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */

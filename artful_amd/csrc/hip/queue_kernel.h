@@ -228,7 +228,7 @@ __device__ __forceinline__ void stampView(const ArtfHipBatchArgs& a, uint32_t vi
 /* views and firstTile come apart from the rest as restrict arguments: nothing the kernel writes can
  * alias them, so their wave-uniform reads are scalar loads. */
 template <int Precision>
-__global__ __launch_bounds__(kQueueGroup) void queueKernel(const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile,
+__global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void queueKernel(const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile,
                                                            ArtfHipBatchArgs a) {
     using Lane = typename LaneOf<Precision>::type;
     const int lane = threadIdx.x & 63;

@@ -188,7 +188,7 @@ def per_view_tiles(native, views, trc, reps=3):
     return {"ms": best, "steps": steps, "lane_slots": slots}
 
 
-def batched_views(native, count=64, seed=4, max_steps=5000, draws=5):
+def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=None):
     """64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
     U[5,85], eye r U[20,100], disc outer U[10,40], each with its own rings and fluid; rays of up to 5000
     steps; one persistent launch replayed from a HIP graph, schedule from the tuner. Beside it, the
@@ -214,7 +214,11 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5):
         b.draw()
         st0, default_ms, _ = run(draws)
         t0 = time.perf_counter()
-        sched = b.tune()
+        if schedule:
+            b.set_schedule(*schedule)
+            sched = b.stats().schedule
+        else:
+            sched = b.tune()
         tune_s = time.perf_counter() - t0
         st, batch_ms, trace_ms = run(draws)
         lat = b.latency()
@@ -288,6 +292,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
     ap.add_argument("--only", choices=["batch", "study", "adjoint"], help="run one extra leg alone and print its JSON")
+    ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
     args = ap.parse_args()
@@ -309,7 +314,9 @@ def main():
         print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 5}))
         return
     if args.only:
-        leg = {"batch": batched_views, "study": precision_study, "adjoint": adjoint_fit}[args.only](native)
+        sched = tuple(int(v) for v in args.batch_schedule.split(",")) if args.batch_schedule else None
+        legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit}
+        leg = legs[args.only](native)
         print(json.dumps({args.only: leg}), flush=True)
         return
     desc, cam = workload_scene(native)
