@@ -196,4 +196,29 @@ void artfHipGraphFree(void* graph) {
     if (graph) (void)hipGraphExecDestroy((hipGraphExec_t)graph);
 }
 
+static bool fluidFits(const ArtfHipFluid* f) {
+    return f && f->density && f->radial && f->around && f->scratch[0] && f->scratch[1] && f->scratch[2] && f->field && f->rings;
+}
+
+int artfHipFluidSeed(const ArtfHipFluid* f, void* stream) {
+    if (!fluidFits(f)) return (int)hipErrorInvalidValue;
+    return (int)launchFluidSeed(*f, (hipStream_t)stream);
+}
+
+int artfHipFluidStir(const ArtfHipFluid* f, const ArtfFluidEddy* eddies, int count, void* stream) {
+    if (!fluidFits(f) || !eddies || count < 0) return (int)hipErrorInvalidValue;
+    if (count == 0) return 0;
+    return (int)launchFluidStir(*f, eddies, count, (hipStream_t)stream);
+}
+
+int artfHipFluidStep(const ArtfHipFluid* f, const ArtfFluidEddy* eddies, void* stream) {
+    if (!fluidFits(f) || !eddies) return (int)hipErrorInvalidValue;
+    return (int)launchFluidStep(*f, eddies, (hipStream_t)stream);
+}
+
+int artfHipFluidField(const ArtfHipFluid* f, void* stream) {
+    if (!fluidFits(f)) return (int)hipErrorInvalidValue;
+    return (int)launchFluidField(*f, (hipStream_t)stream);
+}
+
 } // extern "C"

@@ -29,6 +29,11 @@ int viewsOccupancyExact();
 int viewsOccupancyFast(int precision);
 /* Clears a batch's workspace and stamps its start (post.hip). */
 hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream);
+/* The disc fluid's passes (fluid.hip, contraction off). */
+hipError_t launchFluidSeed(const ArtfHipFluid& f, hipStream_t stream);
+hipError_t launchFluidStir(const ArtfHipFluid& f, const ArtfFluidEddy* eddies, int count, hipStream_t stream);
+hipError_t launchFluidStep(const ArtfHipFluid& f, const ArtfFluidEddy* eddies, hipStream_t stream);
+hipError_t launchFluidField(const ArtfHipFluid& f, hipStream_t stream);
 /* The analytic-disc adjoint (adjoint.hip, contraction off). */
 hipError_t launchAdjoint(const ArtfHipAdjointArgs& a, hipStream_t stream);
 int adjointBlocks(const ArtfHoleFrame& f);

@@ -13,6 +13,7 @@
 #include <stdio.h>
 
 #include "../scene/scene_internal.h"
+#include "artful/device_fluid.h"
 #include "artful_hip.h"
 
 int artfHipCallsLeft = -1;
@@ -157,9 +158,24 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s) {
         h->mapScene = s->id;
         h->mapVersion = hole->mapVersion;
     }
+    if (hole->deviceFluid && hole->fluidDevice == h->device) {
+        /* the scene steps its fluid on this device: read it where it is (its steps are done when the
+         * scene's call returns) */
+        h->fluidScene = 0;
+        return true;
+    }
     if (h->fluidScene != s->id || h->fluidVersion != hole->fluidVersion) {
-        if (!ok(h, artfHipUpload(h->fluid, artfFluidField(hole->fluid), (size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float), h->stream), "fluid upload"))
-            return false;
+        const size_t bytes = (size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float);
+        int status;
+        if (hole->deviceFluid) { /* on another device: through the host */
+            float* field = artfMalloc(bytes);
+            status = artfDeviceFluidRead(hole->deviceFluid, field) ? artfHipUpload(h->fluid, field, bytes, h->stream) : 1;
+            if (!status) status = artfHipStreamSync(h->stream);
+            artfFree(field);
+        } else {
+            status = artfHipUpload(h->fluid, artfFluidField(hole->fluid), bytes, h->stream);
+        }
+        if (!ok(h, status, "fluid upload")) return false;
         h->fluidScene = s->id;
         h->fluidVersion = hole->fluidVersion;
     }
@@ -193,7 +209,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     a.frame = h->frame;
     if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a.frame64);
     a.map = h->map;
-    a.fluid = h->fluid;
+    a.fluid = scene->hole.deviceFluid && scene->hole.fluidDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
     float** out = side == 1 ? h->hdr : h->rays;
     for (int c = 0; c < 3; ++c) a.out[c] = out[c];
     a.counters = h->counters;

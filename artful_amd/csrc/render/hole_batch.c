@@ -1,7 +1,8 @@
 /* artful-hip — a batch of black-hole views traced by one persistent launch (include/artful/batch.h).
  *
  * Made once: the painted map's shared tables (gas, colours, sky, dust) for all views; per view its
- * clamped description, FP32 and FP64 frames, ring tables and (painted disc) fluid, all on the device
+ * clamped description, FP32 and FP64 frames, ring tables and (painted disc) fluid, stepped on the
+ * device (include/artful/device_fluid.h), all on the device
  * with the view table and the queue's tile offsets. Each draw replays one HIP graph: the workspace
  * reset kernel, the persistent trace over all views, and the resolve of each view with several rays a
  * pixel. The tuner times candidate schedules on the batch itself and keeps the fastest. */
@@ -10,6 +11,7 @@
 
 #include "../scene/scene_internal.h"
 #include "artful/batch.h"
+#include "artful/device_fluid.h"
 #include "artful_hip.h"
 
 #define RING_FLOATS ARTF_HIP_RING_FLOATS
@@ -153,12 +155,13 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
         artfDiscRingTables(rows, rows + ARTF_HOLE_MAP_HEIGHT * 2, d.discInner, d.discOuter, d.spin);
         double fluidTime = d.time;
         if (painted) {
-            /* the fluid a scene with this description would hold: made at its time, as artfSceneSetBlackHole does */
-            ArtfFluid* fluid = artfFluidNew(d.discInner, d.discOuter, d.spin, given->hole.time);
-            fluidTime = artfFluidTime(fluid);
-            ready = !artfHipUpload(b->fluids + (size_t)v * FLUID_FLOATS, artfFluidField(fluid), FLUID_FLOATS * sizeof(float), b->stream) &&
+            /* the fluid a scene with this description would hold, made at its time as artfSceneSetBlackHole
+             * does, but stepped on the device (bit for bit the host's) */
+            ArtfDeviceFluid* fluid = artfDeviceFluidNew(d.discInner, d.discOuter, d.spin, given->hole.time, device);
+            ready = fluid && !artfHipCopy(b->fluids + (size_t)v * FLUID_FLOATS, artfDeviceFluidField(fluid), FLUID_FLOATS * sizeof(float), b->stream) &&
                     !artfHipStreamSync(b->stream);
-            artfFluidFree(fluid);
+            if (fluid) fluidTime = artfDeviceFluidTime(fluid);
+            artfDeviceFluidFree(fluid);
         }
         artfHoleFrameOf(&d, &given->camera, fluidTime, bv->width, bv->height, 1, &bv->frame, NULL);
         const size_t pixels = (size_t)bv->width * bv->height;

@@ -18,11 +18,11 @@
 
 #define FW ARTF_FLUID_WIDTH
 #define FH ARTF_FLUID_HEIGHT
-#define FLUID_DT .2f                       /* time per step, in masses */
+#define FLUID_DT ARTF_FLUID_DT
 #define FLUID_DPHI (6.2831853f / (float)FW) /* a cell's angle */
-#define FLUID_CATCH_UP 48                  /* most steps one advance takes */
-#define FLUID_WARM 60                      /* steps before the first field */
-#define FLUID_WOUND 900.f                  /* how long the starting clumps have been sheared */
+#define FLUID_CATCH_UP ARTF_FLUID_CATCH_UP
+#define FLUID_WARM ARTF_FLUID_WARM
+#define FLUID_WOUND ARTF_FLUID_WOUND
 
 struct ArtfFluid {
     float inner, outer;
@@ -31,21 +31,21 @@ struct ArtfFluid {
     float* around[2];  /* w */
     float* field;
     int cur;
-    float r[FH], rowSpan[FH], omega[FH], swing[FH], sound2[FH], shiftCells[FH], envelope[FH], falloff[FH];
+    ArtfFluidRings k;
     uint32_t rng;
     double time;
     ArtfPool* pool;
 };
 
-static uint32_t rngNext(ArtfFluid* f) {
-    uint32_t x = f->rng;
+static uint32_t rngNext(uint32_t* rng) {
+    uint32_t x = *rng;
     x ^= x << 13;
     x ^= x >> 17;
     x ^= x << 5;
-    f->rng = x;
+    *rng = x;
     return x;
 }
-static float rngUnit(ArtfFluid* f) { return (float)(rngNext(f) >> 8) * (1.f / 16777216.f); }
+static float rngUnit(uint32_t* rng) { return (float)(rngNext(rng) >> 8) * (1.f / 16777216.f); }
 
 static float smoothStep(float lo, float hi, float x) {
     const float t = artfClampf((x - lo) / (hi - lo), 0.f, 1.f);
@@ -93,10 +93,10 @@ static void velocityPass(void* user, uint32_t row, uint32_t worker) {
     const float* w = f->around[c];
     float* outVr = f->radial[c ^ 1] + (size_t)j * FW;
     float* outW = f->around[c ^ 1] + (size_t)j * FW;
-    const float r = f->r[j], invR = 1.f / r, cs2 = f->sound2[j], om = f->omega[j], sw = f->swing[j];
-    const float cellsPerW = FLUID_DT / (r * FLUID_DPHI), rowsPerVr = FLUID_DT / f->rowSpan[j];
+    const float r = f->k.r[j], invR = 1.f / r, cs2 = f->k.sound2[j], om = f->k.omega[j], sw = f->k.swing[j];
+    const float cellsPerW = FLUID_DT / (r * FLUID_DPHI), rowsPerVr = FLUID_DT / f->k.rowSpan[j];
     const int jUp = artfMini(j + 1, FH - 1), jDown = artfMaxi(j - 1, 0);
-    const float invSpan = 1.f / (f->r[jUp] - f->r[jDown]), invArc = 1.f / (2.f * r * FLUID_DPHI);
+    const float invSpan = 1.f / (f->k.r[jUp] - f->k.r[jDown]), invArc = 1.f / (2.f * r * FLUID_DPHI);
     const float damp = 1.f - FLUID_DT * .04f * om;
     for (int i = 0; i < FW; ++i) {
         const size_t at = (size_t)j * FW + i;
@@ -121,11 +121,11 @@ static void densityPass(void* user, uint32_t row, uint32_t worker) {
     const float* vr = f->radial[c ^ 1];
     const float* w = f->around[c ^ 1];
     float* out = f->density[c ^ 1] + (size_t)j * FW;
-    const float r = f->r[j], invR = 1.f / r;
-    const float cellsPerW = FLUID_DT / (r * FLUID_DPHI), rowsPerVr = FLUID_DT / f->rowSpan[j];
+    const float r = f->k.r[j], invR = 1.f / r;
+    const float cellsPerW = FLUID_DT / (r * FLUID_DPHI), rowsPerVr = FLUID_DT / f->k.rowSpan[j];
     const int jUp = artfMini(j + 1, FH - 1), jDown = artfMaxi(j - 1, 0);
-    const float rUp = f->r[jUp], rDown = f->r[jDown], invSpan = 1.f / (rUp - rDown), invArc = 1.f / (2.f * r * FLUID_DPHI);
-    const float inflow = -.004f, refill = FLUID_DT * .002f * f->omega[j];
+    const float rUp = f->k.r[jUp], rDown = f->k.r[jDown], invSpan = 1.f / (rUp - rDown), invArc = 1.f / (2.f * r * FLUID_DPHI);
+    const float inflow = -.004f, refill = FLUID_DT * .002f * f->k.omega[j];
     for (int i = 0; i < FW; ++i) {
         const size_t at = (size_t)j * FW + i;
         const float fromX = (float)i - w[at] * cellsPerW, fromY = (float)j - (vr[at] + inflow) * rowsPerVr;
@@ -155,7 +155,7 @@ static void shiftPass(void* user, uint32_t row, uint32_t worker) {
     ArtfFluid* f = user;
     const int from = f->cur ^ 1;
     const size_t at = (size_t)row * FW;
-    const float cells = f->shiftCells[row];
+    const float cells = f->k.shiftCells[row];
     shiftRing(f->density[from] + at, f->density[from ^ 1] + at, cells);
     shiftRing(f->radial[from] + at, f->radial[from ^ 1] + at, cells);
     shiftRing(f->around[from] + at, f->around[from ^ 1] + at, cells);
@@ -166,7 +166,7 @@ static void fieldPass(void* user, uint32_t row, uint32_t worker) {
     ArtfFluid* f = user;
     const float* s = f->density[f->cur] + (size_t)row * FW;
     float* out = f->field + (size_t)row * FW;
-    const float env = f->envelope[row], fall = f->falloff[row];
+    const float env = f->k.envelope[row], fall = f->k.falloff[row];
     for (int i = 0; i < FW; ++i) {
         const float v = s[i] * env;
         out[i] = fall * smoothStep(.1f, 1.1f, v) * artfMinf(v, 2.2f);
@@ -175,41 +175,53 @@ static void fieldPass(void* user, uint32_t row, uint32_t worker) {
 
 /* An eddy at a random ring and cell: a whirl of velocity and a clump of gas (fluid.c:188-216). The
  * random draws are taken in the reference's order: radius, cell, size, sign, strength, clump. */
-static void stirEddy(ArtfFluid* f, bool small) {
+void artfFluidDrawEddy(uint32_t* rng, const ArtfFluidRings* k, float inner, float outer, bool small, ArtfFluidEddy* e) {
+    const float rho = rngUnit(rng), r0 = inner + (outer - inner) * rho * rho;
+    int j0 = 0;
+    while (j0 < FH - 1 && k->r[j0] < r0) ++j0;
+    const int i0 = (int)(rngNext(rng) & (FW - 1));
+    float size;
+    if (small) size = r0 * (.006f + .02f * rngUnit(rng));
+    else size = r0 * (.02f + .1f * rngUnit(rng));
+    const float cs = sqrtf(k->sound2[j0]);
+    float strength = 0.f;
+    if (!small) {
+        const float sign = rngUnit(rng) < .5f ? -1.f : 1.f;
+        const float amount = .3f + .7f * rngUnit(rng);
+        strength = sign * cs * amount;
+    }
+    e->r0 = r0, e->size = size, e->strength = strength;
+    e->clump = (rngUnit(rng) - .42f) * (small ? 1.5f : .5f);
+    e->i0 = i0;
+    e->reach = artfMini((int)(2.5f * size / (r0 * FLUID_DPHI)) + 1, FW / 2 - 1);
+}
+
+static void applyEddy(ArtfFluid* f, const ArtfFluidEddy* e) {
     float* s = f->density[f->cur];
     float* vr = f->radial[f->cur];
     float* w = f->around[f->cur];
-    const float rho = rngUnit(f), r0 = f->inner + (f->outer - f->inner) * rho * rho;
-    int j0 = 0;
-    while (j0 < FH - 1 && f->r[j0] < r0) ++j0;
-    const int i0 = (int)(rngNext(f) & (FW - 1));
-    float size;
-    if (small) size = r0 * (.006f + .02f * rngUnit(f));
-    else size = r0 * (.02f + .1f * rngUnit(f));
-    const float cs = sqrtf(f->sound2[j0]);
-    float strength = 0.f;
-    if (!small) {
-        const float sign = rngUnit(f) < .5f ? -1.f : 1.f;
-        const float amount = .3f + .7f * rngUnit(f);
-        strength = sign * cs * amount;
-    }
-    const float clump = (rngUnit(f) - .42f) * (small ? 1.5f : .5f);
+    const float r0 = e->r0, size = e->size, strength = e->strength, clump = e->clump;
     const float invSize2 = 1.f / (size * size);
-    const int reach = artfMini((int)(2.5f * size / (r0 * FLUID_DPHI)) + 1, FW / 2 - 1);
     for (int j = 0; j < FH; ++j) {
-        const float dr = f->r[j] - r0;
+        const float dr = f->k.r[j] - r0;
         if (dr * dr > 6.25f * size * size) continue;
-        for (int di = -reach; di <= reach; ++di) {
-            const float dx = (float)di * f->r[j] * FLUID_DPHI;
+        for (int di = -e->reach; di <= e->reach; ++di) {
+            const float dx = (float)di * f->k.r[j] * FLUID_DPHI;
             const float d2 = (dr * dr + dx * dx) * invSize2;
             if (d2 > 6.25f) continue;
             const float g = 1.f - d2 * .16f, shape = g * g * g;
-            const size_t at = (size_t)j * FW + (size_t)((i0 + di) & (FW - 1));
+            const size_t at = (size_t)j * FW + (size_t)((e->i0 + di) & (FW - 1));
             vr[at] -= strength * shape * dx / size;
             w[at] += strength * shape * dr / size;
             s[at] = artfClampf(s[at] * (1.f + clump * shape), .03f, 8.f);
         }
     }
+}
+
+static void stirEddy(ArtfFluid* f, bool small) {
+    ArtfFluidEddy e;
+    artfFluidDrawEddy(&f->rng, &f->k, f->inner, f->outer, small, &e);
+    applyEddy(f, &e);
 }
 
 /* Each ring's mean drawn 3% of the way back to 1 (fluid.c:221-229). */
@@ -245,11 +257,31 @@ void artfFluidFree(ArtfFluid* f) {
     artfFree(f);
 }
 
+void artfFluidRingsOf(float inner, float outer, float spin, ArtfFluidRings* out) {
+    const float k = 1.f - inner / outer;
+    for (int j = 0; j < FH; ++j) {
+        const float rho = (float)j / (float)(FH - 1), r = inner / (1.f - rho * k);
+        out->r[j] = r;
+        const float root = sqrtf(r), om = 1.f / (r * root + spin);
+        out->omega[j] = om;
+        /* Kerr's epicyclic frequency squared: zero at the last stable orbit */
+        const float kappa2 = om * om * (1.f - 6.f / r + 8.f * spin / (r * root) - 3.f * spin * spin / (r * r));
+        out->swing[j] = artfMaxf(kappa2, -.5f * om * om) / (2.f * om);
+        const float cs = .03f * r * om; /* sound: the disc's thickness times the orbit's speed */
+        out->sound2[j] = cs * cs;
+        out->shiftCells[j] = om * FLUID_DT / FLUID_DPHI;
+        out->envelope[j] = smoothStep(inner + .5f, inner + 2.2f, r) * (1.f - .86f * smoothStep(outer * .12f, outer * .8f, r));
+        out->falloff[j] = sqrtf(inner / r);
+    }
+    for (int j = 0; j < FH; ++j)
+        out->rowSpan[j] = (out->r[artfMini(j + 1, FH - 1)] - out->r[artfMaxi(j - 1, 0)]) * (j == 0 || j == FH - 1 ? 1.f : .5f);
+}
+
 ArtfFluid* artfFluidNew(float inner, float outer, float spin, double time) {
     ArtfFluid* f = ARTF_NEW(ArtfFluid);
     f->inner = inner;
     f->outer = outer;
-    f->rng = 0x9E3779B9u;
+    f->rng = ARTF_FLUID_SEED;
     f->time = time;
     f->pool = artfPoolNew(0);
     const size_t cells = (size_t)FW * FH;
@@ -260,28 +292,12 @@ ArtfFluid* artfFluidNew(float inner, float outer, float spin, double time) {
     }
     f->field = artfMalloc(cells * 4);
 
-    const float k = 1.f - inner / outer;
-    for (int j = 0; j < FH; ++j) {
-        const float rho = (float)j / (float)(FH - 1), r = inner / (1.f - rho * k);
-        f->r[j] = r;
-        const float root = sqrtf(r), om = 1.f / (r * root + spin);
-        f->omega[j] = om;
-        /* Kerr's epicyclic frequency squared: zero at the last stable orbit */
-        const float kappa2 = om * om * (1.f - 6.f / r + 8.f * spin / (r * root) - 3.f * spin * spin / (r * r));
-        f->swing[j] = artfMaxf(kappa2, -.5f * om * om) / (2.f * om);
-        const float cs = .03f * r * om; /* sound: the disc's thickness times the orbit's speed */
-        f->sound2[j] = cs * cs;
-        f->shiftCells[j] = om * FLUID_DT / FLUID_DPHI;
-        f->envelope[j] = smoothStep(inner + .5f, inner + 2.2f, r) * (1.f - .86f * smoothStep(outer * .12f, outer * .8f, r));
-        f->falloff[j] = sqrtf(inner / r);
-    }
-    for (int j = 0; j < FH; ++j)
-        f->rowSpan[j] = (f->r[artfMini(j + 1, FH - 1)] - f->r[artfMaxi(j - 1, 0)]) * (j == 0 || j == FH - 1 ? 1.f : .5f);
+    artfFluidRingsOf(inner, outer, spin, &f->k);
 
     /* clumps already wound up by FLUID_WOUND of shear: each ring turned by its own orbit */
     float* s = f->density[0];
     for (int j = 0; j < FH; ++j) {
-        const float rho = (float)j / (float)(FH - 1), wound = f->omega[j] * FLUID_WOUND / 6.2831853f;
+        const float rho = (float)j / (float)(FH - 1), wound = f->k.omega[j] * FLUID_WOUND / 6.2831853f;
         for (int i = 0; i < FW; ++i) {
             float turn = (float)i / (float)FW - wound;
             turn -= floorf(turn);

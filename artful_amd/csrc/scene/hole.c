@@ -7,6 +7,8 @@
  * geometry out in doubles, for the FP64 and mixed-precision tracers. */
 #include "scene_internal.h"
 
+#include "artful/device_fluid.h"
+
 static double cubeRootNewton(double x) {
     if (x <= 0) return 0;
     double v = x > 1 ? x : 1;
@@ -64,6 +66,19 @@ ArtfBlackHoleDesc artfHoleClamped(const ArtfBlackHoleDesc* desc) {
     return d;
 }
 
+/* A new fluid for the hole's disc at this moment: on the host, or on the scene's fluid device (falling
+ * back to the host, logged, if that device cannot hold it). */
+static void newFluid(ArtfHole* h, double time) {
+    const float inner = h->desc.discInner, outer = h->desc.discOuter, spin = h->desc.spin;
+    artfFluidFree(h->fluid);
+    artfDeviceFluidFree(h->deviceFluid);
+    h->fluid = NULL, h->deviceFluid = NULL;
+    if (h->fluidDevice >= 0 && !(h->deviceFluid = artfDeviceFluidNew(inner, outer, spin, time, h->fluidDevice)))
+        artfLog("artful: the fluid stays on the host");
+    if (!h->deviceFluid) h->fluid = artfFluidNew(inner, outer, spin, time);
+    ++h->fluidVersion;
+}
+
 void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     ArtfHole* h = &s->hole;
     if (!desc) {
@@ -83,12 +98,23 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
         }
         artfDiscRings(h->map, inner, outer, spin);
         ++h->mapVersion;
-        artfFluidFree(h->fluid);
-        h->fluid = artfFluidNew(inner, outer, spin, desc->time);
-        ++h->fluidVersion;
-    } else if (artfFluidAdvance(h->fluid, desc->time)) {
+        newFluid(h, desc->time);
+    } else if (h->deviceFluid ? artfDeviceFluidAdvance(h->deviceFluid, desc->time) : artfFluidAdvance(h->fluid, desc->time)) {
         ++h->fluidVersion;
     }
+}
+
+double artfHoleFluidTime(const ArtfHole* h) {
+    if (h->deviceFluid) return artfDeviceFluidTime(h->deviceFluid);
+    return h->fluid ? artfFluidTime(h->fluid) : 0.0;
+}
+
+void artfSceneSetFluidOnDevice(ArtfScene* s, int device) {
+    ArtfHole* h = &s->hole;
+    if (device < 0) device = -1;
+    if (device == h->fluidDevice) return;
+    h->fluidDevice = device;
+    if (h->map) newFluid(h, h->desc.time); /* the disc's gas starts again, in its new place */
 }
 
 /* The eye's local frame and the Kerr metric there, in either precision. HOLE_REAL is float or double;
@@ -240,7 +266,7 @@ void artfHoleFrame64At(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double s
 }
 
 void artfHoleFrameLevels(const ArtfScene* s, int width, int height, int levelsCap, ArtfHoleFrame* out, uint32_t* bloomPixels) {
-    const double t = s->hole.fluid ? artfFluidTime(s->hole.fluid) : 0.0;
+    const double t = artfHoleFluidTime(&s->hole);
     artfHoleFrameOf(&s->hole.desc, &s->camera, t, width, height, levelsCap, out, bloomPixels);
 }
 

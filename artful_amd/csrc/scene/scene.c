@@ -2,6 +2,8 @@
  * as far as the hole's path goes). */
 #include "scene_internal.h"
 
+#include "artful/device_fluid.h"
+
 #include <stdatomic.h>
 
 #include "artful/inspect.h"
@@ -14,6 +16,7 @@ ArtfScene* artfSceneNew(void) {
     s->camera.near_ = .1f;
     s->camera.up = artfVec3(0, 1, 0);
     s->camera.position = artfVec3(0, 0, 1);
+    s->hole.fluidDevice = -1;
     return s;
 }
 
@@ -21,6 +24,7 @@ void artfSceneFree(ArtfScene* s) {
     if (!s) return;
     artfFree(s->hole.map);
     artfFluidFree(s->hole.fluid);
+    artfDeviceFluidFree(s->hole.deviceFluid);
     artfFree(s);
 }
 
@@ -39,7 +43,7 @@ const float* artfSceneHoleMap(const ArtfScene* s, uint64_t* version) {
 
 const float* artfSceneHoleFluid(const ArtfScene* s, uint64_t* version, double* time) {
     if (version) *version = s->hole.fluidVersion;
-    if (time) *time = s->hole.fluid ? artfFluidTime(s->hole.fluid) : 0.0;
+    if (time) *time = artfHoleFluidTime(&s->hole);
     return s->hole.fluid ? artfFluidField(s->hole.fluid) : NULL;
 }
 

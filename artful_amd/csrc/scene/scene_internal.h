@@ -14,9 +14,14 @@ typedef struct ArtfHole {
     ArtfBlackHoleDesc desc;
     float* map;            /* ARTF_HOLE_MAP_FLOATS */
     uint64_t mapVersion;
-    ArtfFluid* fluid;
+    ArtfFluid* fluid;      /* the host fluid, or */
+    struct ArtfDeviceFluid* deviceFluid; /* the same fluid stepped on HIP device fluidDevice (>= 0) */
+    int fluidDevice;
     uint64_t fluidVersion;
 } ArtfHole;
+
+/* The time of the scene's fluid, wherever it is (0 without one). */
+double artfHoleFluidTime(const ArtfHole* h);
 
 struct ArtfScene {
     uint64_t id;
@@ -31,7 +36,13 @@ void artfDiscRings(float* map, float inner, float outer, float spin);
 void artfDiscRingTables(float* rows, float* glow, float inner, float outer, float spin);
 
 /* ---- fluid.c (Artful src/scene/fluid.c) ---- */
+#define ARTF_FLUID_CATCH_UP 48 /* most steps one advance takes */
+#define ARTF_FLUID_WARM 60     /* steps before the first field */
+#define ARTF_FLUID_SEED 0x9E3779B9u /* the random stream's first word */
 ArtfFluid* artfFluidNew(float inner, float outer, float spin, double time);
+/* The ring constants of a fluid on this disc; and the next eddy of a fluid's random stream (rng). */
+void artfFluidRingsOf(float inner, float outer, float spin, ArtfFluidRings* out);
+void artfFluidDrawEddy(uint32_t* rng, const ArtfFluidRings* k, float inner, float outer, bool small, ArtfFluidEddy* e);
 void artfFluidFree(ArtfFluid* f);
 bool artfFluidAdvance(ArtfFluid* f, double time);
 const float* artfFluidField(const ArtfFluid* f);

@@ -83,6 +83,10 @@ def lib() -> C.CDLL:
         "artfHoleBatchLatency": ([P, P], B),
         "artfHoleBatchSetSchedule": ([P, C.POINTER(abi.HoleSchedule)], B),
         "artfHoleBatchTune": ([P, I, C.POINTER(abi.HoleSchedule)], B),
+        "artfDeviceFluidNew": ([F, F, F, D, I], P), "artfDeviceFluidFree": ([P], None),
+        "artfDeviceFluidAdvance": ([P, D], B), "artfDeviceFluidField": ([P], P),
+        "artfDeviceFluidRead": ([P, P], B), "artfDeviceFluidTime": ([P], D),
+        "artfSceneSetFluidOnDevice": ([P, I], None),
     }
     for name, (args, ret) in sig.items():
         fn = getattr(L, name)
@@ -140,6 +144,10 @@ class Scene:
     def set_camera(self, cam: abi.Camera) -> None:
         self._L.artfSceneSetCamera(self.ptr, C.byref(cam))
 
+    def set_fluid_on_device(self, device: int) -> None:
+        """Step the disc's fluid on HIP device `device` (-1: on the host, the default)."""
+        self._L.artfSceneSetFluidOnDevice(self.ptr, device)
+
     def set_hole(self, desc: abi.BlackHoleDesc | None) -> None:
         self._L.artfSceneSetBlackHole(self.ptr, C.byref(desc) if desc is not None else None)
 
@@ -160,6 +168,8 @@ class Scene:
     def fluid(self) -> tuple[np.ndarray, int, float]:
         v, t = C.c_uint64(), C.c_double()
         p = self._L.artfSceneHoleFluid(self.ptr, C.byref(v), C.byref(t))
+        if not p:  # no disc, or its fluid on a device
+            return None, v.value, t.value
         return np.ctypeslib.as_array(p, shape=(abi.FLUID_WIDTH * abi.FLUID_HEIGHT,)).copy(), v.value, t.value
 
     def frame(self, width: int, height: int, levels: int = 0) -> tuple[abi.HoleFrame, int]:
@@ -389,6 +399,39 @@ class HoleBatch:
     def close(self) -> None:
         if self.ptr:
             self._L.artfHoleBatchFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceFluid:
+    """The disc fluid stepped on the GPU (include/artful/device_fluid.h)."""
+
+    def __init__(self, inner: float, outer: float, spin: float, time: float = 0.0, device: int = 0):
+        self._L = lib()
+        self.ptr = self._L.artfDeviceFluidNew(inner, outer, spin, time, device)
+        if not self.ptr:
+            raise RuntimeError("artfDeviceFluidNew failed")
+
+    def advance(self, time: float) -> bool:
+        return bool(self._L.artfDeviceFluidAdvance(self.ptr, time))
+
+    def field(self) -> np.ndarray:
+        out = np.zeros(abi.FLUID_WIDTH * abi.FLUID_HEIGHT, np.float32)
+        if not self._L.artfDeviceFluidRead(self.ptr, out.ctypes.data):
+            raise RuntimeError("artfDeviceFluidRead failed")
+        return out
+
+    def time(self) -> float:
+        return self._L.artfDeviceFluidTime(self.ptr)
+
+    def close(self) -> None:
+        if self.ptr:
+            self._L.artfDeviceFluidFree(self.ptr)
             self.ptr = None
 
     def __del__(self):

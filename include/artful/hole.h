@@ -20,6 +20,9 @@ extern "C" {
 #define ARTF_HOLE_DUST_HEIGHT 256
 #define ARTF_FLUID_WIDTH 1024 /* cells around a ring (a power of two) */
 #define ARTF_FLUID_HEIGHT 256 /* rings */
+#define ARTF_FLUID_DT .2f     /* the fluid's time a step, in masses */
+#define ARTF_FLUID_WOUND 900.f /* how long its starting clumps have been sheared */
+#define ARTF_FLUID_STEP_EDDIES 16 /* eddies stirred in a step: 2 large, then 14 small */
 #define ARTF_HOLE_COLORS 256
 #define ARTF_HOLE_SKY_WIDTH 1024
 #define ARTF_HOLE_SKY_HEIGHT 512
@@ -54,6 +57,22 @@ typedef enum ArtfDiscModel {
     ARTF_DISC_PAINTED = 0,  /* the reference's disc: painted map x fluid x dust */
     ARTF_DISC_ANALYTIC = 1  /* a thin disc of emissivity (inner / r)^q, no map, fluid or dust */
 } ArtfDiscModel;
+
+/* The disc fluid's per-ring constants (Artful src/scene/fluid.c:255-285): radius, the span a ring's
+ * cells cover in r, orbital rate, Kerr epicyclic swing, sound speed squared, cells a ring turns a step,
+ * the envelope and falloff of what shows. Worked out on the host, the same for the CPU and GPU steps. */
+typedef struct ArtfFluidRings {
+    float r[ARTF_FLUID_HEIGHT], rowSpan[ARTF_FLUID_HEIGHT], omega[ARTF_FLUID_HEIGHT], swing[ARTF_FLUID_HEIGHT];
+    float sound2[ARTF_FLUID_HEIGHT], shiftCells[ARTF_FLUID_HEIGHT], envelope[ARTF_FLUID_HEIGHT], falloff[ARTF_FLUID_HEIGHT];
+} ArtfFluidRings;
+
+/* One eddy stirred into the fluid (fluid.c:188-216): centred at radius r0 and cell i0, of the given
+ * size, spinning with strength, adding clump to the gas; it touches cells within reach around. Its
+ * parameters come from the fluid's random stream on the host; applying it is the same on CPU and GPU. */
+typedef struct ArtfFluidEddy {
+    float r0, size, strength, clump;
+    int32_t i0, reach;
+} ArtfFluidEddy;
 
 /* Per-frame uniforms, in units of the hole's mass. */
 typedef struct ArtfHoleFrame {

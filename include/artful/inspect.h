@@ -15,7 +15,8 @@ extern "C" {
 bool artfSceneHoleDesc(const ArtfScene* s, ArtfBlackHoleDesc* out);
 /* ARTF_HOLE_MAP_FLOATS floats, and the version that changes when they do; NULL before any hole. */
 const float* artfSceneHoleMap(const ArtfScene* s, uint64_t* version);
-/* ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats, their version, and the moment they are of. */
+/* ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats, their version, and the moment they are of; NULL (version and
+ * moment still given) when the fluid is stepped on a device (artfSceneSetFluidOnDevice). */
 const float* artfSceneHoleFluid(const ArtfScene* s, uint64_t* version, double* time);
 /* The frame's uniforms at this size (levels: 0 for the reference's bloom rule, else at most that many);
  * returns the floats each bloom plane needs. */

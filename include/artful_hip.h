@@ -154,6 +154,30 @@ int artfHipGraphEnd(void* stream, void** graph);
 int artfHipGraphLaunch(void* graph, void* stream);
 void artfHipGraphFree(void* graph);
 
+/* ---- the disc fluid on the GPU: replaces the host passes of artfFluidNew / artfFluidAdvance
+ * (src/scene/fluid.c:96-241, 255-336), bit for bit. The host keeps the random stream and the ring
+ * constants (artfFluidRingsOf, artfFluidDrawEddy in the scene code) and hands over eddies. ---- */
+
+typedef struct ArtfHipFluid {
+    /* device, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats each, ring by ring: the state ... */
+    float* density;
+    float* radial;
+    float* around;
+    float* scratch[3]; /* ... a step's second buffers (density, radial, around) ... */
+    float* field;      /* ... and what shows, for the tracer */
+    const ArtfFluidRings* rings; /* device */
+} ArtfHipFluid;
+
+/* The starting clumps (fluid.c:286-305); velocities zero. */
+int artfHipFluidSeed(const ArtfHipFluid* fluid, void* stream);
+/* Eddies (device, count of them) stirred in, in their order. */
+int artfHipFluidStir(const ArtfHipFluid* fluid, const ArtfFluidEddy* eddies, int count, void* stream);
+/* One step (fluid.c:231-241): its ARTF_FLUID_STEP_EDDIES eddies (device), then the velocity, density,
+ * shift and balance passes. */
+int artfHipFluidStep(const ArtfHipFluid* fluid, const ArtfFluidEddy* eddies, void* stream);
+/* field from the state (fluid.c:164-176). */
+int artfHipFluidField(const ArtfHipFluid* fluid, void* stream);
+
 /* ---- the adjoint (not in the reference): image-loss gradients of the analytic-disc tracer ---- */
 
 typedef struct ArtfHipAdjointArgs {
