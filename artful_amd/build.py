@@ -26,10 +26,11 @@ ARCH = os.environ.get("ARTF_OFFLOAD_ARCH", "gfx950")
 
 C_SOURCES = ["core/base.c", "core/pool.c", "scene/disc.c", "scene/fluid.c", "scene/hole.c", "scene/scene.c",
              "render/renderer.c", "render/hip_backend.c", "render/hole_adjoint.c", "render/hole_batch.c",
-             "render/device_fluid.c"]
+             "render/device_disc.c"]
 # (source, contraction)
 HIP_SOURCES = [("hip/trace_exact.hip", "off"), ("hip/trace_fast.hip", "fast"), ("hip/post.hip", "off"),
-               ("hip/adjoint.hip", "off"), ("hip/fluid.hip", "off"), ("hip/abi.hip", "off")]
+               ("hip/adjoint.hip", "off"), ("hip/fluid.hip", "off"), ("hip/disc.hip", "off"),
+               ("hip/abi.hip", "off")]
 
 C_FLAGS = ["-std=c11", "-O3", "-fPIC", "-Wall", "-Wno-parentheses", "-ffp-contract=off", "-fno-math-errno",
            "-fno-trapping-math", f"-I{ROOT / 'include'}", f"-I{CSRC}"]

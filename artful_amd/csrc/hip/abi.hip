@@ -221,4 +221,9 @@ int artfHipFluidField(const ArtfHipFluid* f, void* stream) {
     return (int)launchFluidField(*f, (hipStream_t)stream);
 }
 
+int artfHipDiscPaint(float* map, void* stream) {
+    if (!map) return (int)hipErrorInvalidValue;
+    return (int)launchDiscPaint(map, (hipStream_t)stream);
+}
+
 } // extern "C"

@@ -1,16 +1,47 @@
-/* artful-hip — the disc fluid on the GPU (include/artful/device_fluid.h): the host's fluid
- * (scene/fluid.c) with its passes moved to the kernels of hip/fluid.hip. The random stream, the clock
- * and the ring constants stay here and are the host fluid's own code (artfFluidRingsOf,
- * artfFluidDrawEddy), so the eddies handed to the device are the very ones the host would stir. */
+/* artful-hip — the disc's data on the GPU (include/artful/device_disc.h).
+ *
+ * The map: gas, dust and sky painted by hip/disc.hip; the colour table and the ring rows and glow are
+ * the host's own code (artfDiscColors, artfDiscRingTables), a few thousand floats, uploaded.
+ * The fluid: the host's fluid (scene/fluid.c) with its passes moved to the kernels of hip/fluid.hip.
+ * The random stream, the clock and the ring constants stay here and are the host fluid's own code
+ * (artfFluidRingsOf, artfFluidDrawEddy), so the eddies handed to the device are the very ones the host
+ * would stir. */
 #include <stdio.h>
 #include <string.h>
 
 #include "../scene/scene_internal.h"
-#include "artful/device_fluid.h"
+#include "artful/device_disc.h"
 #include "artful_hip.h"
 
 #define CELLS ((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT)
 #define FIRST_STIRS 200 /* large eddies before the warm-up steps (fluid.c:306) */
+
+float* artfDeviceMapNew(int device) {
+    if (device < 0 || device >= artfHipDeviceCount() || artfHipSetDevice(device)) {
+        artfLog("artful: no HIP device %d for the disc map", device);
+        return NULL;
+    }
+    const size_t bytes = (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float);
+    float* map = artfHipMalloc(bytes);
+    float colors[ARTF_HOLE_COLORS * 3];
+    artfDiscColors(colors);
+    if (!map || artfHipMemset(map, 0, bytes, NULL) || artfHipDiscPaint(map, NULL) ||
+        artfHipUpload(map + ARTF_HOLE_COLORS_AT, colors, sizeof colors, NULL) || artfHipStreamSync(NULL)) {
+        artfLog("artful: the disc map could not be painted on the device");
+        artfHipFree(map);
+        return NULL;
+    }
+    return map;
+}
+
+bool artfDeviceMapRings(float* map, float inner, float outer, float spin, int device) {
+    float rows[ARTF_HOLE_MAP_HEIGHT * 2], glow[ARTF_HOLE_MAP_HEIGHT];
+    artfDiscRingTables(rows, glow, inner, outer, spin);
+    return !artfHipSetDevice(device) && !artfHipUpload(map + ARTF_HOLE_ROWS_AT, rows, sizeof rows, NULL) &&
+           !artfHipUpload(map + ARTF_HOLE_GLOW_AT, glow, sizeof glow, NULL) && !artfHipStreamSync(NULL);
+}
+
+void artfDeviceMapFree(float* map) { artfHipFree(map); }
 
 struct ArtfDeviceFluid {
     int device;

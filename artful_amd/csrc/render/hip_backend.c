@@ -13,7 +13,7 @@
 #include <stdio.h>
 
 #include "../scene/scene_internal.h"
-#include "artful/device_fluid.h"
+#include "artful/device_disc.h"
 #include "artful_hip.h"
 
 int artfHipCallsLeft = -1;
@@ -153,12 +153,25 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s) {
         h->fluid = artfHipMalloc((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float));
         if (!h->map || !h->fluid) return ok(h, 2, "allocate");
     }
-    if (h->mapScene != s->id || h->mapVersion != hole->mapVersion) {
-        if (!ok(h, artfHipUpload(h->map, hole->map, (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float), h->stream), "map upload")) return false;
+    if (hole->deviceMap && hole->discDevice == h->device) {
+        h->mapScene = 0; /* the scene's map is on this device: read in place */
+    } else if (h->mapScene != s->id || h->mapVersion != hole->mapVersion) {
+        const size_t bytes = (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float);
+        int status;
+        if (hole->deviceMap) { /* on another device: through the host */
+            float* staged = artfMalloc(bytes);
+            status = artfHipDownload(staged, hole->deviceMap, bytes, NULL);
+            if (!status) status = artfHipUpload(h->map, staged, bytes, h->stream);
+            if (!status) status = artfHipStreamSync(h->stream);
+            artfFree(staged);
+        } else {
+            status = artfHipUpload(h->map, hole->map, bytes, h->stream);
+        }
+        if (!ok(h, status, "map upload")) return false;
         h->mapScene = s->id;
         h->mapVersion = hole->mapVersion;
     }
-    if (hole->deviceFluid && hole->fluidDevice == h->device) {
+    if (hole->deviceFluid && hole->discDevice == h->device) {
         /* the scene steps its fluid on this device: read it where it is (its steps are done when the
          * scene's call returns) */
         h->fluidScene = 0;
@@ -208,8 +221,8 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     memset(&a, 0, sizeof a);
     a.frame = h->frame;
     if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a.frame64);
-    a.map = h->map;
-    a.fluid = scene->hole.deviceFluid && scene->hole.fluidDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
+    a.map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
+    a.fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
     float** out = side == 1 ? h->hdr : h->rays;
     for (int c = 0; c < 3; ++c) a.out[c] = out[c];
     a.counters = h->counters;

@@ -1,8 +1,8 @@
 /* artful-hip — a batch of black-hole views traced by one persistent launch (include/artful/batch.h).
  *
- * Made once: the painted map's shared tables (gas, colours, sky, dust) for all views; per view its
- * clamped description, FP32 and FP64 frames, ring tables and (painted disc) fluid, stepped on the
- * device (include/artful/device_fluid.h), all on the device
+ * Made once, on the device (include/artful/device_disc.h): the painted map's shared tables (gas,
+ * colours, sky, dust) for all views; per view its clamped description, FP32 and FP64 frames, ring
+ * tables and (painted disc) fluid
  * with the view table and the queue's tile offsets. Each draw replays one HIP graph: the workspace
  * reset kernel, the persistent trace over all views, and the resolve of each view with several rays a
  * pixel. The tuner times candidate schedules on the batch itself and keeps the fastest. */
@@ -11,7 +11,7 @@
 
 #include "../scene/scene_internal.h"
 #include "artful/batch.h"
-#include "artful/device_fluid.h"
+#include "artful/device_disc.h"
 #include "artful_hip.h"
 
 #define RING_FLOATS ARTF_HIP_RING_FLOATS
@@ -63,7 +63,7 @@ void artfHoleBatchFree(ArtfHoleBatch* b) {
     if (b->views)
         for (int v = 0; v < b->count; ++v) freePlanes(b->views[v].hdr), freePlanes(b->views[v].rays);
     artfFree(b->views);
-    artfHipFree(b->map), artfHipFree(b->rings), artfHipFree(b->fluids);
+    artfDeviceMapFree(b->map), artfHipFree(b->rings), artfHipFree(b->fluids);
     artfHipFree(b->deviceViews), artfHipFree(b->firstTile), artfHipFree(b->workspace);
     artfFree(b->countersHost), artfFree(b->stampsHost);
     for (int i = 0; i < 2; ++i) artfHipEventDestroy(b->events[i]);
@@ -133,17 +133,15 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     for (int i = 0; i < 2; ++i) ready = ready && (b->events[i] = artfHipEventCreate()) != NULL;
     if (!ready) return fail(b, "make its stream");
 
-    /* the shared tables, painted once */
-    float* map = artfMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
-    artfDiscPaint(map);
+    /* the shared tables, painted once on the device */
     const bool painted = t.disc == ARTF_DISC_PAINTED;
     float* rings = artfMalloc((size_t)count * RING_FLOATS * sizeof(float));
     ArtfHipView* hv = artfCalloc((size_t)count, sizeof(ArtfHipView));
     uint32_t* firstTile = artfMalloc((size_t)(count + 1) * sizeof(uint32_t));
-    b->map = artfHipMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
+    b->map = artfDeviceMapNew(device);
     b->rings = artfHipMalloc((size_t)count * RING_FLOATS * sizeof(float));
     b->fluids = painted ? artfHipMalloc((size_t)count * FLUID_FLOATS * sizeof(float)) : NULL;
-    ready = b->map && b->rings && (b->fluids || !painted) && !artfHipUpload(b->map, map, (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float), b->stream);
+    ready = b->map && b->rings && (b->fluids || !painted);
 
     uint32_t at = 0;
     for (int v = 0; ready && v < count; ++v) {
@@ -192,7 +190,7 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
                 !artfHipUpload(b->firstTile, firstTile, (size_t)(count + 1) * sizeof(uint32_t), b->stream) && !artfHipStreamSync(b->stream);
     }
     artfHipStreamSync(b->stream); /* the uploads have read the host copies */
-    artfFree(map), artfFree(rings), artfFree(hv), artfFree(firstTile);
+    artfFree(rings), artfFree(hv), artfFree(firstTile);
     if (!ready) return fail(b, "set up its device memory");
 
     char* w = (char*)b->workspace;

@@ -7,7 +7,7 @@
  * geometry out in doubles, for the FP64 and mixed-precision tracers. */
 #include "scene_internal.h"
 
-#include "artful/device_fluid.h"
+#include "artful/device_disc.h"
 
 static double cubeRootNewton(double x) {
     if (x <= 0) return 0;
@@ -66,17 +66,38 @@ ArtfBlackHoleDesc artfHoleClamped(const ArtfBlackHoleDesc* desc) {
     return d;
 }
 
-/* A new fluid for the hole's disc at this moment: on the host, or on the scene's fluid device (falling
+/* A new fluid for the hole's disc at this moment: on the host, or on the scene's disc device (falling
  * back to the host, logged, if that device cannot hold it). */
 static void newFluid(ArtfHole* h, double time) {
     const float inner = h->desc.discInner, outer = h->desc.discOuter, spin = h->desc.spin;
     artfFluidFree(h->fluid);
     artfDeviceFluidFree(h->deviceFluid);
     h->fluid = NULL, h->deviceFluid = NULL;
-    if (h->fluidDevice >= 0 && !(h->deviceFluid = artfDeviceFluidNew(inner, outer, spin, time, h->fluidDevice)))
+    if (h->discDevice >= 0 && !(h->deviceFluid = artfDeviceFluidNew(inner, outer, spin, time, h->discDevice)))
         artfLog("artful: the fluid stays on the host");
     if (!h->deviceFluid) h->fluid = artfFluidNew(inner, outer, spin, time);
     ++h->fluidVersion;
+}
+
+/* The map for the hole's disc: painted once (on the disc device, or on the host as the reference
+ * does, also when that device cannot hold it), then this disc's ring tables. */
+static void ringMap(ArtfHole* h) {
+    const float inner = h->desc.discInner, outer = h->desc.discOuter, spin = h->desc.spin;
+    if (h->discDevice >= 0 && !h->deviceMap && !h->map && !(h->deviceMap = artfDeviceMapNew(h->discDevice)))
+        artfLog("artful: the disc map stays on the host");
+    if (h->deviceMap && !artfDeviceMapRings(h->deviceMap, inner, outer, spin, h->discDevice)) {
+        artfLog("artful: the disc map goes back to the host");
+        artfDeviceMapFree(h->deviceMap);
+        h->deviceMap = NULL;
+    }
+    if (!h->deviceMap) {
+        if (!h->map) {
+            h->map = artfMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
+            artfDiscPaint(h->map);
+        }
+        artfDiscRings(h->map, inner, outer, spin);
+    }
+    ++h->mapVersion;
 }
 
 void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
@@ -87,17 +108,12 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     }
     const ArtfBlackHoleDesc clamped = artfHoleClamped(desc);
     const float spin = clamped.spin, inner = clamped.discInner, outer = clamped.discOuter;
-    const bool sameDisc = h->map && h->desc.discInner == inner && h->desc.discOuter == outer;
+    const bool sameDisc = (h->map || h->deviceMap) && h->desc.discInner == inner && h->desc.discOuter == outer;
     const bool sameSpin = sameDisc && h->desc.spin == spin;
     h->on = true;
     h->desc = clamped;
     if (!sameSpin) {
-        if (!h->map) {
-            h->map = artfMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
-            artfDiscPaint(h->map);
-        }
-        artfDiscRings(h->map, inner, outer, spin);
-        ++h->mapVersion;
+        ringMap(h);
         newFluid(h, desc->time);
     } else if (h->deviceFluid ? artfDeviceFluidAdvance(h->deviceFluid, desc->time) : artfFluidAdvance(h->fluid, desc->time)) {
         ++h->fluidVersion;
@@ -109,12 +125,19 @@ double artfHoleFluidTime(const ArtfHole* h) {
     return h->fluid ? artfFluidTime(h->fluid) : 0.0;
 }
 
-void artfSceneSetFluidOnDevice(ArtfScene* s, int device) {
+void artfSceneSetDiscOnDevice(ArtfScene* s, int device) {
     ArtfHole* h = &s->hole;
     if (device < 0) device = -1;
-    if (device == h->fluidDevice) return;
-    h->fluidDevice = device;
-    if (h->map) newFluid(h, h->desc.time); /* the disc's gas starts again, in its new place */
+    if (device == h->discDevice) return;
+    const bool had = h->map || h->deviceMap;
+    h->discDevice = device;
+    artfFree(h->map);
+    artfDeviceMapFree(h->deviceMap);
+    h->map = NULL, h->deviceMap = NULL;
+    if (had) { /* the disc again, in its new place; its gas starts again */
+        ringMap(h);
+        newFluid(h, h->desc.time);
+    }
 }
 
 /* The eye's local frame and the Kerr metric there, in either precision. HOLE_REAL is float or double;

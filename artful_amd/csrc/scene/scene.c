@@ -2,7 +2,7 @@
  * as far as the hole's path goes). */
 #include "scene_internal.h"
 
-#include "artful/device_fluid.h"
+#include "artful/device_disc.h"
 
 #include <stdatomic.h>
 
@@ -16,7 +16,7 @@ ArtfScene* artfSceneNew(void) {
     s->camera.near_ = .1f;
     s->camera.up = artfVec3(0, 1, 0);
     s->camera.position = artfVec3(0, 0, 1);
-    s->hole.fluidDevice = -1;
+    s->hole.discDevice = -1;
     return s;
 }
 
@@ -25,6 +25,7 @@ void artfSceneFree(ArtfScene* s) {
     artfFree(s->hole.map);
     artfFluidFree(s->hole.fluid);
     artfDeviceFluidFree(s->hole.deviceFluid);
+    artfDeviceMapFree(s->hole.deviceMap);
     artfFree(s);
 }
 

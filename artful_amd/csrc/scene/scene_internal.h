@@ -12,11 +12,12 @@ typedef struct ArtfFluid ArtfFluid;
 typedef struct ArtfHole {
     bool on;
     ArtfBlackHoleDesc desc;
-    float* map;            /* ARTF_HOLE_MAP_FLOATS */
+    float* map;            /* ARTF_HOLE_MAP_FLOATS on the host, or */
+    float* deviceMap;      /* the same map in the memory of HIP device discDevice (>= 0) */
     uint64_t mapVersion;
     ArtfFluid* fluid;      /* the host fluid, or */
-    struct ArtfDeviceFluid* deviceFluid; /* the same fluid stepped on HIP device fluidDevice (>= 0) */
-    int fluidDevice;
+    struct ArtfDeviceFluid* deviceFluid; /* the same fluid stepped on device discDevice */
+    int discDevice;
     uint64_t fluidVersion;
 } ArtfHole;
 

@@ -86,7 +86,8 @@ def lib() -> C.CDLL:
         "artfDeviceFluidNew": ([F, F, F, D, I], P), "artfDeviceFluidFree": ([P], None),
         "artfDeviceFluidAdvance": ([P, D], B), "artfDeviceFluidField": ([P], P),
         "artfDeviceFluidRead": ([P, P], B), "artfDeviceFluidTime": ([P], D),
-        "artfSceneSetFluidOnDevice": ([P, I], None),
+        "artfSceneSetDiscOnDevice": ([P, I], None),
+        "artfDeviceMapNew": ([I], P), "artfDeviceMapFree": ([P], None), "artfDeviceMapRings": ([P, F, F, F, I], B),
     }
     for name, (args, ret) in sig.items():
         fn = getattr(L, name)
@@ -144,9 +145,9 @@ class Scene:
     def set_camera(self, cam: abi.Camera) -> None:
         self._L.artfSceneSetCamera(self.ptr, C.byref(cam))
 
-    def set_fluid_on_device(self, device: int) -> None:
-        """Step the disc's fluid on HIP device `device` (-1: on the host, the default)."""
-        self._L.artfSceneSetFluidOnDevice(self.ptr, device)
+    def set_disc_on_device(self, device: int) -> None:
+        """Build the disc's map and step its fluid on HIP device `device` (-1: on the host, the default)."""
+        self._L.artfSceneSetDiscOnDevice(self.ptr, device)
 
     def set_hole(self, desc: abi.BlackHoleDesc | None) -> None:
         self._L.artfSceneSetBlackHole(self.ptr, C.byref(desc) if desc is not None else None)
@@ -155,9 +156,11 @@ class Scene:
         d = abi.BlackHoleDesc()
         return d if self._L.artfSceneHoleDesc(self.ptr, C.byref(d)) else None
 
-    def map(self) -> np.ndarray:
+    def map(self) -> np.ndarray | None:
         v = C.c_uint64()
         p = self._L.artfSceneHoleMap(self.ptr, C.byref(v))
+        if not p:  # no disc, or its map on a device
+            return None
         return np.ctypeslib.as_array(p, shape=(abi.MAP_FLOATS,)).copy()
 
     def map_version(self) -> int:
@@ -409,7 +412,7 @@ class HoleBatch:
 
 
 class DeviceFluid:
-    """The disc fluid stepped on the GPU (include/artful/device_fluid.h)."""
+    """The disc fluid stepped on the GPU (include/artful/device_disc.h)."""
 
     def __init__(self, inner: float, outer: float, spin: float, time: float = 0.0, device: int = 0):
         self._L = lib()
@@ -439,4 +442,21 @@ class DeviceFluid:
             self.close()
         except Exception:
             pass
+
+
+def device_map(spin: float, inner: float, outer: float, device: int = 0) -> np.ndarray:
+    """A disc map painted on the GPU (artfDeviceMapNew, then this disc's rings), copied to the host."""
+    L = lib()
+    m = L.artfDeviceMapNew(device)
+    if not m:
+        raise RuntimeError("artfDeviceMapNew failed")
+    try:
+        if not L.artfDeviceMapRings(m, inner, outer, spin, device):
+            raise RuntimeError("artfDeviceMapRings failed")
+        out = np.zeros(abi.MAP_FLOATS, np.float32)
+        if L.artfHipDownload(out.ctypes.data, m, out.nbytes, None):
+            raise RuntimeError("download failed")
+        return out
+    finally:
+        L.artfDeviceMapFree(m)
 

@@ -13,10 +13,11 @@ extern "C" {
 
 /* The hole's description after artfSceneSetBlackHole's clamps; false when the scene has no hole. */
 bool artfSceneHoleDesc(const ArtfScene* s, ArtfBlackHoleDesc* out);
-/* ARTF_HOLE_MAP_FLOATS floats, and the version that changes when they do; NULL before any hole. */
+/* ARTF_HOLE_MAP_FLOATS floats, and the version that changes when they do; NULL before any hole, and
+ * when the map is in device memory (artfSceneSetDiscOnDevice). */
 const float* artfSceneHoleMap(const ArtfScene* s, uint64_t* version);
 /* ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats, their version, and the moment they are of; NULL (version and
- * moment still given) when the fluid is stepped on a device (artfSceneSetFluidOnDevice). */
+ * moment still given) when the fluid is stepped on a device (artfSceneSetDiscOnDevice). */
 const float* artfSceneHoleFluid(const ArtfScene* s, uint64_t* version, double* time);
 /* The frame's uniforms at this size (levels: 0 for the reference's bloom rule, else at most that many);
  * returns the floats each bloom plane needs. */

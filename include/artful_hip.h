@@ -154,6 +154,13 @@ int artfHipGraphEnd(void* stream, void** graph);
 int artfHipGraphLaunch(void* graph, void* stream);
 void artfHipGraphFree(void* graph);
 
+/* ---- the disc map on the GPU: replaces buildDisc (src/scene/hole.c:136-178), bit for bit ---- */
+
+/* Paints the map's gas (normalised), the disc's dust and the sky's dust into a device map of
+ * ARTF_HOLE_MAP_FLOATS; the colour table and the per-ring rows and glow are the host's to fill
+ * (artfDiscColors / artfDiscRingTables in the scene code: a few thousand floats). */
+int artfHipDiscPaint(float* map, void* stream);
+
 /* ---- the disc fluid on the GPU: replaces the host passes of artfFluidNew / artfFluidAdvance
  * (src/scene/fluid.c:96-241, 255-336), bit for bit. The host keeps the random stream and the ring
  * constants (artfFluidRingsOf, artfFluidDrawEddy in the scene code) and hands over eddies. ---- */

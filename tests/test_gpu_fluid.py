@@ -1,4 +1,4 @@
-"""The disc fluid stepped on the GPU (include/artful/device_fluid.h) against the reference's fluid.
+"""The disc fluid stepped on the GPU (include/artful/device_disc.h) against the reference's fluid.
 
 The fluid is chaotic: one wrong bit anywhere grows into a different flow within a few steps. So the
 GPU's field must equal the reference's digests bit for bit (the standalone fluid at five moments, each
@@ -65,13 +65,14 @@ def test_long_run_equals_host(gpu, disc):
         dev.close()
 
 
-def test_renderer_with_the_fluid_on_the_device(gpu):
-    """A scene whose fluid steps on the GPU draws the host-fluid scene's frames bit for bit, through an
-    animation (a step a frame, as the reference's drivers advance time), a jump and a new disc."""
+def test_renderer_with_the_disc_on_the_device(gpu):
+    """A scene whose map is painted and whose fluid steps on the GPU draws the host scene's frames bit
+    for bit, through an animation (a step a frame, as the reference's drivers advance time), a jump and
+    a new disc."""
     from tests.golden import cases
     desc, cam = cases.make(cases.CASES["advance"])
     host, dev = gpu.Scene(desc, cam), gpu.Scene(cam=cam)
-    dev.set_fluid_on_device(0)
+    dev.set_disc_on_device(0)
     dev.set_hole(desc)
     r = gpu.Renderer(96, 60)
     try:
