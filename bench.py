@@ -242,6 +242,47 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=No
     return out
 
 
+def animation(native, frames=30):
+    """The reference driver's loop (tools/blackhole.c: time advanced .2 a frame, one fluid step) on the
+    1080p view: artfSceneSetBlackHole then artfRendererDraw, per frame, wall clock, with the disc's map
+    and fluid on the host (the reference's way: the fluid stepped by the CPU, then uploaded) and on the
+    device (artfSceneSetDiscOnDevice); also the scene's first set-up (map paint + fluid warm-up) each
+    way, and whether the two give the same frames."""
+    desc, cam = workload_scene(native)
+    r = native.Renderer(WIDTH, HEIGHT)
+    r.set_tracing("fp32")
+    out = {"frames": frames}
+    last = {}
+    for where in ("host", "device"):
+        t0 = time.perf_counter()
+        s = native.Scene(cam=cam)
+        if where == "device":
+            s.set_disc_on_device(0)
+        desc.time = 0.0
+        s.set_hole(desc)
+        setup = time.perf_counter() - t0
+        r.draw(s)
+        t0 = time.perf_counter()
+        hole_ms = 0.0
+        for k in range(1, frames + 1):
+            desc.time = .2 * k
+            t1 = time.perf_counter()
+            s.set_hole(desc)
+            hole_ms += (time.perf_counter() - t1) * 1e3
+            if not r.draw(s):
+                raise RuntimeError("draw failed")
+        wall = time.perf_counter() - t0
+        last[where] = r.read_hdr()
+        out[where] = {"setup_s": setup, "ms_per_frame": wall / frames * 1e3, "set_hole_ms_per_frame": hole_ms / frames,
+                      "trace_ms": r.hole_stats().traceMs}
+        s.close()
+        log(f"animation {where}: {out[where]}")
+    r.close()
+    out["same_frames"] = bool(np.array_equal(last["host"].view(np.uint32), last["device"].view(np.uint32)))
+    out["speedup"] = out["host"]["ms_per_frame"] / out["device"]["ms_per_frame"]
+    return out
+
+
 def precision_study(native, width=3840, height=2160):
     """3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
     75, 89}, eye at r 50, painted disc ISCO..20, Doppler 1: FP32, mixed and FP64 traces, each image's
@@ -291,7 +332,7 @@ def main():
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
-    ap.add_argument("--only", choices=["batch", "study", "adjoint"], help="run one extra leg alone and print its JSON")
+    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation"], help="run one extra leg alone and print its JSON")
     ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
@@ -315,7 +356,8 @@ def main():
         return
     if args.only:
         sched = tuple(int(v) for v in args.batch_schedule.split(",")) if args.batch_schedule else None
-        legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit}
+        legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit,
+                "animation": animation}
         leg = legs[args.only](native)
         print(json.dumps({args.only: leg}), flush=True)
         return
@@ -350,6 +392,7 @@ def main():
     log(f"adjoint: {adjoint}")
     renderer.close()
     scene.close()
+    anim = animation(native)
     batch = None if args.no_batch else batched_views(native)
     study = None if args.no_study else precision_study(native)
 
@@ -372,6 +415,7 @@ def main():
         "rel_error_vs_fp64": errors,
         "adjoint_over_forward": adjoint["adjoint_over_forward"],
         "adjoint": adjoint,
+        "animation": anim,
         "batched_views": batch,
         "precision_study": study,
     }
