@@ -105,7 +105,8 @@ class HipSchedule(C.Structure):
 
 
 class HipBatchArgs(C.Structure):
-    _fields_ = [("views", C.c_void_p), ("firstTile", C.c_void_p), ("viewCount", C.c_int32), ("tiles", C.c_uint32),
+    _fields_ = [("views", C.c_void_p), ("firstTile", C.c_void_p), ("viewCount", C.c_int32), ("tileBegin", C.c_uint32),
+                ("tileEnd", C.c_uint32),
                 ("precision", C.c_int32), ("map", C.c_void_p), ("rings", C.c_void_p), ("fluids", C.c_void_p),
                 ("queue", C.c_void_p), ("counters", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule)]
 
@@ -114,12 +115,14 @@ class HoleView(C.Structure):
     _fields_ = [("hole", BlackHoleDesc), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32)]
 
 
-HoleSchedule = HipSchedule  # the same four fields (include/artful/batch.h)
+class HoleSchedule(C.Structure):
+    _fields_ = [("workgroups", C.c_int32), ("chunkTiles", C.c_int32), ("refill", C.c_int32), ("stepsPerCheck", C.c_int32),
+                ("launchTiles", C.c_int32)]
 
 
 class HoleBatchStats(C.Structure):
     _fields_ = [("batchMs", C.c_double), ("traceMs", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64),
-                ("laneSlots", C.c_uint64), ("views", C.c_int32), ("schedule", HipSchedule)]
+                ("laneSlots", C.c_uint64), ("views", C.c_int32), ("schedule", HoleSchedule)]
 
 
 # enum values (include/artful/renderer.h, include/artful/hole.h)

@@ -137,7 +137,8 @@ int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const blo
 }
 
 static bool batchFits(const ArtfHipBatchArgs* a) {
-    if (!a || !a->views || !a->firstTile || !a->map || !a->rings || !a->queue || a->viewCount < 1 || a->tiles < 1 || a->tiles > (1u << 25))
+    if (!a || !a->views || !a->firstTile || !a->map || !a->rings || !a->queue || a->viewCount < 1 || a->tileBegin >= a->tileEnd ||
+        a->tileEnd > (1u << 25))
         return false;
     if (a->viewCount > 65536) return false;
     const ArtfHipSchedule& s = a->schedule;
@@ -154,6 +155,11 @@ int artfHipBatchReset(const ArtfHipBatchArgs* a, void* stream) {
 
 /* The views themselves live on the device, where the host cannot check them: the caller (the batch in
  * render/hole_batch.c) builds them from frames it has checked with frameFits and uploads them. */
+int artfHipBatchResetQueues(const ArtfHipBatchArgs* a, void* stream) {
+    if (!batchFits(a)) return (int)hipErrorInvalidValue;
+    return (int)launchQueueReset(*a, (hipStream_t)stream);
+}
+
 int artfHipTraceViews(const ArtfHipBatchArgs* a, void* stream) {
     if (!batchFits(a)) return (int)hipErrorInvalidValue;
     if (a->precision == ARTF_PRECISION_FP32) return (int)launchViewsExact(*a, (hipStream_t)stream);

@@ -29,6 +29,7 @@ int viewsOccupancyExact();
 int viewsOccupancyFast(int precision);
 /* Clears a batch's workspace and stamps its start (post.hip). */
 hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream);
+hipError_t launchQueueReset(const ArtfHipBatchArgs& a, hipStream_t stream);
 /* The disc map's painted gas, dust and sky (disc.hip, contraction off). */
 hipError_t launchDiscPaint(float* map, hipStream_t stream);
 /* The disc fluid's passes (fluid.hip, contraction off). */

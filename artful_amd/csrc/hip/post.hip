@@ -178,6 +178,15 @@ __global__ __launch_bounds__(256) void batchResetKernel(uint32_t* queue, unsigne
     if (stamps && i == 0) stamps[0] = wall_clock64();
 }
 
+__global__ __launch_bounds__(64) void queueResetKernel(uint32_t* queue) {
+    if (threadIdx.x < ARTF_HIP_QUEUES) queue[threadIdx.x * ARTF_HIP_QUEUE_STRIDE] = 0u;
+}
+
+hipError_t launchQueueReset(const ArtfHipBatchArgs& a, hipStream_t stream) {
+    hipLaunchKernelGGL(queueResetKernel, dim3(1), dim3(64), 0, stream, a.queue);
+    return hipGetLastError();
+}
+
 hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream) {
     const int stamps = 1 + a.viewCount * ARTF_HIP_STAMP_SLOTS;
     const int n = max(2 * ARTF_HIP_COUNTER_SLOTS, stamps);

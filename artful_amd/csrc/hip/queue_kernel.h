@@ -200,8 +200,10 @@ template <int Precision> struct LaneOf { using type = PlainLane<float>; };
 template <> struct LaneOf<ARTF_PRECISION_FP64> { using type = PlainLane<double>; };
 template <> struct LaneOf<ARTF_PRECISION_MIXED> { using type = MixedLane; };
 
-/* The first tile of queue q's share: q * tiles / ARTF_HIP_QUEUES. */
-__device__ __forceinline__ uint32_t segmentEdge(uint32_t tiles, uint32_t q) { return (uint32_t)(((uint64_t)tiles * q) / ARTF_HIP_QUEUES); }
+/* The first tile of queue q's share of the launch's tiles. */
+__device__ __forceinline__ uint32_t segmentEdge(const ArtfHipBatchArgs& a, uint32_t q) {
+    return a.tileBegin + (uint32_t)(((uint64_t)(a.tileEnd - a.tileBegin) * q) / ARTF_HIP_QUEUES);
+}
 
 /* The view holding tile t: the last v with firstTile[v] <= t (a wave-uniform search). */
 __device__ __forceinline__ uint32_t viewOfTile(const uint32_t* __restrict__ firstTile, int viewCount, uint32_t tile) {
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
                     for (int t = 0; t < ARTF_HIP_QUEUES && !got; ++t) {
                         const uint32_t q = (xcd + (uint32_t)t) % ARTF_HIP_QUEUES;
                         if (empty & (1u << q)) continue;
-                        const uint32_t begin = segmentEdge(a.tiles, q), end = segmentEdge(a.tiles, q + 1);
+                        const uint32_t begin = segmentEdge(a, q), end = segmentEdge(a, q + 1);
                         uint32_t base = 0;
                         if (lane == 0) base = atomicAdd(&a.queue[q * ARTF_HIP_QUEUE_STRIDE], (uint32_t)chunk);
                         base = uniform(base);

@@ -40,6 +40,7 @@ struct ArtfHoleBatch {
     unsigned long long* countersHost;
     unsigned long long* stampsHost;
     uint64_t rays;
+    uint32_t tiles;
     ArtfHoleSchedule schedule;
     void* graph;
     double batchMs, clockHz;
@@ -80,6 +81,7 @@ static ArtfHoleSchedule resolved(const ArtfHoleBatch* b, const ArtfHoleSchedule*
     r.chunkTiles = r.chunkTiles <= 0 ? 2 : artfClampi(r.chunkTiles, 1, 64);
     r.refill = r.refill <= 0 ? 64 : artfClampi(r.refill, 1, 64);
     r.stepsPerCheck = r.stepsPerCheck <= 0 ? 8 : artfClampi(r.stepsPerCheck, 1, 16);
+    r.launchTiles = r.launchTiles < 0 ? 0 : r.launchTiles;
     return r;
 }
 
@@ -195,7 +197,8 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
 
     char* w = (char*)b->workspace;
     ArtfHipBatchArgs* a = &b->args;
-    a->views = b->deviceViews, a->firstTile = b->firstTile, a->viewCount = count, a->tiles = at;
+    a->views = b->deviceViews, a->firstTile = b->firstTile, a->viewCount = count, a->tileBegin = 0, a->tileEnd = at;
+    b->tiles = at;
     a->precision = t.precision, a->map = b->map, a->rings = b->rings, a->fluids = b->fluids;
     a->queue = (uint32_t*)w;
     a->counters = (unsigned long long*)(w + queueBytes);
@@ -206,14 +209,21 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     return b;
 }
 
-/* The graph of one draw: the reset, the trace over all views, each view's resolve. */
+/* The graph of one draw: the reset, the trace over all views (in slices of launchTiles tiles, the
+ * queues cleared between them), each view's resolve. */
 static bool capture(ArtfHoleBatch* b) {
     ArtfHipBatchArgs* a = &b->args;
     a->schedule.workgroups = b->schedule.workgroups, a->schedule.chunkTiles = b->schedule.chunkTiles;
     a->schedule.refill = b->schedule.refill, a->schedule.stepsPerCheck = b->schedule.stepsPerCheck;
+    const uint32_t slice = b->schedule.launchTiles > 0 ? (uint32_t)b->schedule.launchTiles : b->tiles;
     if (artfHipGraphBegin(b->stream)) return false;
+    a->tileBegin = 0, a->tileEnd = slice < b->tiles ? slice : b->tiles;
     int status = artfHipBatchReset(a, b->stream);
-    if (!status) status = artfHipTraceViews(a, b->stream);
+    for (uint32_t begin = 0; !status && begin < b->tiles; begin += slice) {
+        a->tileBegin = begin, a->tileEnd = b->tiles - begin > slice ? begin + slice : b->tiles;
+        if (begin > 0) status = artfHipBatchResetQueues(a, b->stream);
+        if (!status) status = artfHipTraceViews(a, b->stream);
+    }
     for (int v = 0; !status && v < b->count; ++v)
         if (b->views[v].side > 1) status = artfHipResolve(&b->views[v].frame, b->views[v].rays, b->views[v].hdr, b->stream);
     void* graph = NULL;

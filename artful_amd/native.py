@@ -388,8 +388,8 @@ class HoleBatch:
             raise RuntimeError("artfHoleBatchLatency failed")
         return out
 
-    def set_schedule(self, workgroups=0, chunk_tiles=0, refill=0, steps_per_check=0) -> None:
-        s = abi.HoleSchedule(workgroups, chunk_tiles, refill, steps_per_check)
+    def set_schedule(self, workgroups=0, chunk_tiles=0, refill=0, steps_per_check=0, launch_tiles=0) -> None:
+        s = abi.HoleSchedule(workgroups, chunk_tiles, refill, steps_per_check, launch_tiles)
         if not self._L.artfHoleBatchSetSchedule(self.ptr, C.byref(s)):
             raise RuntimeError("artfHoleBatchSetSchedule failed")
 

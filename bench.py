@@ -283,6 +283,39 @@ def animation(native, frames=30):
     return out
 
 
+def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, draws=3):
+    """SURVEY.md (f)4's huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays, 6.4 GB of ray planes)
+    as a batch of one view in slices of 2^20 tiles (67 M rays, ~8 ms a launch: watchdog-sized), beside
+    the renderer's single launch of the tile kernel."""
+    desc, cam = workload_scene(native, samples=side)
+    out = {"width": width, "height": height, "rays_per_pixel": side * side, "rays": width * height * side * side,
+           "launch_tiles": launch_tiles}
+    b = native.HoleBatch([(desc, cam, width, height)], native.tracing(precision="fp32", disc="painted"))
+    b.set_schedule(0, 2, 64, 8, launch_tiles)
+    ms = []
+    for _ in range(draws + 1):
+        b.draw()
+        ms.append(b.stats().batchMs)
+    st = b.stats()
+    out["batch"] = {"ms": float(np.median(ms[1:])), "rays_per_s": st.rays / (np.median(ms[1:]) * 1e-3),
+                    "launches": -(-((width + 7) // 8 * ((height + 7) // 8) * side * side) // launch_tiles),
+                    "steps_per_ray": st.steps / st.rays}
+    b.close()
+    r = native.Renderer(width, height)
+    s = native.Scene(desc, cam)
+    r.set_tracing("fp32")
+    tr = []
+    for _ in range(draws + 1):
+        if not r.draw(s):
+            raise RuntimeError("draw failed")
+        tr.append(r.hole_stats().traceMs)
+    out["renderer_trace"] = {"ms": float(np.median(tr[1:])), "rays_per_s": out["rays"] / (np.median(tr[1:]) * 1e-3)}
+    r.close()
+    s.close()
+    log(f"huge frame: {out}")
+    return out
+
+
 def precision_study(native, width=3840, height=2160):
     """3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
     75, 89}, eye at r 50, painted disc ISCO..20, Doppler 1: FP32, mixed and FP64 traces, each image's
@@ -332,7 +365,7 @@ def main():
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
-    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation"], help="run one extra leg alone and print its JSON")
+    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge"], help="run one extra leg alone and print its JSON")
     ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
@@ -357,7 +390,7 @@ def main():
     if args.only:
         sched = tuple(int(v) for v in args.batch_schedule.split(",")) if args.batch_schedule else None
         legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit,
-                "animation": animation}
+                "animation": animation, "huge": huge_frame}
         leg = legs[args.only](native)
         print(json.dumps({args.only: leg}), flush=True)
         return
@@ -393,6 +426,7 @@ def main():
     renderer.close()
     scene.close()
     anim = animation(native)
+    huge = huge_frame(native)
     batch = None if args.no_batch else batched_views(native)
     study = None if args.no_study else precision_study(native)
 
@@ -416,6 +450,7 @@ def main():
         "adjoint_over_forward": adjoint["adjoint_over_forward"],
         "adjoint": adjoint,
         "animation": anim,
+        "huge_frame": huge,
         "batched_views": batch,
         "precision_study": study,
     }

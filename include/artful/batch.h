@@ -23,12 +23,15 @@ typedef struct ArtfHoleView {
     int32_t width, height;
 } ArtfHoleView;
 
-/* How the persistent launch spends the device (see artful_hip.h's ArtfHipSchedule). */
+/* How the persistent launch spends the device (the first four fields as artful_hip.h's ArtfHipSchedule). */
 typedef struct ArtfHoleSchedule {
     int32_t workgroups;    /* 0: all the device holds at once */
     int32_t chunkTiles;    /* 8x8 tiles a wave takes from the queue at once */
     int32_t refill;        /* idle lanes of 64 that make a wave take new rays */
     int32_t stepsPerCheck; /* steps between two looks at the idle lanes */
+    /* 0: all tiles in one launch; else launches of at most this many 8x8 tiles each, one after another
+     * in the graph (a huge batch in watchdog-sized slices) */
+    int32_t launchTiles;
 } ArtfHoleSchedule;
 
 typedef struct ArtfHoleBatchStats {

@@ -115,17 +115,20 @@ typedef struct ArtfHipBatchArgs {
     const uint32_t* firstTile; /* device, viewCount + 1: view v's tiles are [firstTile[v], firstTile[v + 1]), each
                                 * view's tiles sub-ray by sub-ray, rows of tiles top to bottom */
     int32_t viewCount;
-    uint32_t tiles;            /* firstTile[viewCount] */
+    /* the tiles this launch traces, [tileBegin, tileEnd) of [0, firstTile[viewCount]): a huge batch is
+     * traced by several launches of a slice each (watchdog-sized), the queues cleared between them */
+    uint32_t tileBegin, tileEnd;
     int32_t precision;         /* ArtfPrecision, the same for every view */
     const float* map;          /* device, ARTF_HOLE_MAP_FLOATS: its gas, colours, sky and dust are shared */
     const float* rings;        /* device, viewCount * ARTF_HIP_RING_FLOATS: view v's at v * ARTF_HIP_RING_FLOATS */
     /* device, viewCount * ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT: view v's fluid field at v times that; read
      * for painted views only (NULL if there are none) */
     const float* fluids;
-    /* device workspace, cleared by artfHipBatchReset before each launch: ARTF_HIP_QUEUES counters, each
-     * ARTF_HIP_QUEUE_STRIDE words from the last. Queue q hands out the tiles [q * tiles / ARTF_HIP_QUEUES,
-     * (q + 1) * tiles / ARTF_HIP_QUEUES) to the workgroups of its XCD (blockIdx % ARTF_HIP_QUEUES, the
-     * order workgroups are dealt to XCDs), which then take from the other queues until all are empty. */
+    /* device workspace, cleared by artfHipBatchReset (or, between the slices of one batch,
+     * artfHipBatchResetQueues) before each launch: ARTF_HIP_QUEUES counters, each ARTF_HIP_QUEUE_STRIDE
+     * words from the last. Queue q hands out the q-th eighth of the launch's tiles to the workgroups of
+     * its XCD (blockIdx % ARTF_HIP_QUEUES, the order workgroups are dealt to XCDs), which then take from
+     * the other queues until all are empty. */
     uint32_t* queue;
     /* device, 2 * ARTF_HIP_COUNTER_SLOTS, as artfHipTrace's counters (lane-slots: every lane of every
      * stepping wave, idle ones included); NULL: not counted */
@@ -141,6 +144,8 @@ typedef struct ArtfHipBatchArgs {
 /* Clears the queue, the counters and the stamps, and stamps the start, with a kernel (no
  * memset node, so that the pair can be captured in a graph and replayed). */
 int artfHipBatchReset(const ArtfHipBatchArgs* args, void* stream);
+/* Clears the queues alone, between two launches of one batch (the counters and stamps run on). */
+int artfHipBatchResetQueues(const ArtfHipBatchArgs* args, void* stream);
 int artfHipTraceViews(const ArtfHipBatchArgs* args, void* stream);
 /* Workgroups of the persistent kernel for this precision one compute unit holds at once. */
 int artfHipViewsOccupancy(int precision);

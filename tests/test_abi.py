@@ -28,7 +28,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
     assert C.sizeof(abi.HipTraceArgs) == 800
-    assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 96 and C.sizeof(abi.HoleBatchStats) == 64
+    assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 96 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
     assert abi.MAP_FLOATS == 2887424
 
 
@@ -72,14 +72,16 @@ def test_batch_rejects_bad_arguments(native):
     a = abi.HipBatchArgs()
     assert L.artfHipTraceViews(C.byref(a), None) != 0 and L.artfHipBatchReset(C.byref(a), None) != 0
     a.views = a.firstTile = a.map = a.rings = a.queue = 64
-    a.viewCount, a.tiles, a.precision = 1, 1, 0
+    a.viewCount, a.tileBegin, a.tileEnd, a.precision = 1, 0, 1, 0
     for bad in [(0, 1, 1, 1), (1, 0, 1, 1), (1, 65, 1, 1), (1, 1, 0, 1), (1, 1, 65, 1), (1, 1, 1, 0), (1, 1, 1, 17)]:
         a.schedule = abi.HipSchedule(*bad)
         assert L.artfHipTraceViews(C.byref(a), None) != 0, bad
     a.schedule = abi.HipSchedule(1, 1, 1, 1)
     a.viewCount = 70000  # more views than a batch holds
     assert L.artfHipTraceViews(C.byref(a), None) != 0
-    a.viewCount, a.precision = 1, 4
+    a.viewCount, a.tileBegin = 1, 1  # an empty slice
+    assert L.artfHipTraceViews(C.byref(a), None) != 0
+    a.tileBegin, a.precision = 0, 4
     assert L.artfHipTraceViews(C.byref(a), None) != 0
 
     desc, cam = native.hole_desc(), native.camera((0, 10, 50))

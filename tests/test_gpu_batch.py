@@ -91,9 +91,10 @@ def test_fp32_analytic_long_rays_equals_renderer(gpu, renderer):
     print("steps per ray", stats.steps / stats.rays, "active-lane fraction", stats.steps / stats.laneSlots)
 
 
-@pytest.mark.parametrize("schedule", [(1, 1, 64, 1), (3, 5, 1, 3), (700, 64, 17, 16), (0, 2, 8, 2)])
+@pytest.mark.parametrize("schedule", [(1, 1, 64, 1), (3, 5, 1, 3), (700, 64, 17, 16), (0, 2, 8, 2), (0, 2, 64, 8, 100), (5, 3, 20, 2, 7)])
 def test_schedules_give_the_same_image(gpu, renderer, schedule):
-    """One workgroup alone, odd chunks and refill thresholds, more workgroups than rays."""
+    """One workgroup alone, odd chunks and refill thresholds, more workgroups than rays, the batch in
+    slices of 100 and of 7 tiles (one launch each, queues cleared between)."""
     tracing = dict(precision="fp32", disc="painted")
     images, stats = batch_images(gpu, SMALL, schedule=schedule, **tracing)
     for k, view in enumerate(SMALL):
@@ -153,3 +154,13 @@ def test_c_abi_rejects_bad_schedules(gpu):
     L = gpu.lib()
     a = abi.HipBatchArgs()
     assert L.artfHipTraceViews(a, None) != 0  # no views
+
+
+def test_huge_frame_in_slices(gpu, renderer):
+    """A frame of 8x8 rays a pixel (640 x 360: 14.7 M rays) as a batch of one view in slices of 20,000
+    tiles (12 launches), against the renderer's one launch."""
+    view = view_of(.998, 60, 60, 30, 640, 360, fov_y=.35, samples=8)
+    tracing = dict(precision="fp32", disc="painted")
+    images, stats = batch_images(gpu, [view], schedule=(0, 2, 64, 8, 20000), **tracing)
+    assert stats.rays == 640 * 360 * 64
+    assert same_bits(images[0], rendered(gpu, renderer, view, **tracing))
