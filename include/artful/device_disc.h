@@ -5,7 +5,7 @@
 #ifndef ARTFUL_DEVICE_DISC_H
 #define ARTFUL_DEVICE_DISC_H
 
-#include "artful/core.h"
+#include "artful/scene.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -40,7 +40,6 @@ double artfDeviceFluidTime(const ArtfDeviceFluid* f);
  * no host work beyond the small tables and no upload; -1 (the default) keeps them on the host, as the
  * reference does. A scene that already has a disc builds it again in the new place (its gas starts
  * again at the description's time). Falls back to the host, logged, if the device cannot hold them. */
-typedef struct ArtfScene ArtfScene;
 void artfSceneSetDiscOnDevice(ArtfScene* scene, int device);
 
 #ifdef __cplusplus
