@@ -47,6 +47,20 @@ int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream) {
     return (int)hipStreamSynchronize((hipStream_t)stream);
 }
 
+int artfHipDownloadAsync(void* dst, const void* src, size_t bytes, void* stream) {
+    return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+}
+
+void* artfHipHostMalloc(size_t bytes) {
+    void* p = NULL;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return NULL;
+    return p;
+}
+
+void artfHipHostFree(void* p) {
+    if (p) (void)hipHostFree(p);
+}
+
 int artfHipMemset(void* dst, int value, size_t bytes, void* stream) {
     return (int)hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
 }

@@ -6,6 +6,12 @@
  * uniforms from artfHoleFrame), and per frame runs trace -> (resolve) -> bloom -> tone map on one
  * stream, timed with events between the stages. It is plain C over include/artful_hip.h.
  *
+ * A draw only queues its frame (its step counts copied to pinned memory by the stream itself); the
+ * frame is waited for by what needs it: reading pixels or HDR back, the stats, the next draw's host
+ * work never. So frames follow one another on the device without the host in between, as the Vulkan
+ * backend's frames in flight do (vulkan.c vkDraw waits on a fence some frames old). A device fault in
+ * a queued frame is reported by the call that waits for it.
+ *
  * Device memory per renderer: the map (11.5 MB) and fluid (1 MB) once; per size, the HDR planes
  * (12 B/px), the rays' planes when a pixel has several (12 B/ray), the bloom pyramid, and BGRA. */
 #include "renderer_internal.h"
@@ -44,9 +50,11 @@ typedef struct HipRenderer {
     uint32_t* bgra;
     size_t bgraPixels;
     unsigned long long* counters;     /* device: 2 * ARTF_HIP_COUNTER_SLOTS */
-    unsigned long long* countersHost; /* their copy, summed on the host */
+    unsigned long long* countersHost; /* their copy (pinned), summed on the host once the frame is done */
     /* the last frame */
     bool drawn, hole;
+    bool queued; /* its stats still to be read off the device */
+    size_t frameRays;
     uint32_t background;
     ArtfHoleFrame frame;
     ArtfHoleStats holeStats;
@@ -114,8 +122,13 @@ static void hipSetOverlay(ArtfRenderer* r, const ArtfOverlay* o) {
     for (int y = 0; y < o->height; ++y) memcpy(h->overlay + (size_t)y * o->width, o->pixels + (size_t)y * stride, (size_t)o->width * 4);
 }
 
+static bool settle(HipRenderer* h);
+
 static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
     const size_t pixels = (size_t)h->width * h->height;
+    /* a queued frame may still use what is about to be replaced */
+    const bool replace = pixels != h->hdrPixels || (rays > pixels && rays > h->rayCount) || bloomPixels > h->bloomPixels;
+    if (replace && h->queued && !settle(h)) return false;
     if (pixels != h->hdrPixels) {
         freePlanes(h->hdr);
         artfHipFree(h->bgra);
@@ -243,24 +256,37 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     if (!ok(h, artfHipEventRecord(h->events[EV_BLOOMED], h->stream), "event")) return false;
     if (!ok(h, artfHipToneMap(&h->frame, h->hdr, h->bloom, h->bgra, h->stream), "tone map")) return false;
     if (!ok(h, artfHipEventRecord(h->events[EV_TONED], h->stream), "event")) return false;
-    unsigned long long* slots = h->countersHost;
-    if (!ok(h, artfHipDownload(slots, h->counters, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *slots, h->stream), "wait")) return false;
+    if (!ok(h, artfHipDownloadAsync(h->countersHost, h->counters, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost, h->stream), "queue"))
+        return false;
+    (void)tUp;
+    h->frameRays = rays;
+    h->queued = true;
+    h->drawn = true;
+    /* a scene whose disc lives on the device changes it on streams of its own when it is next set:
+     * its frame is finished before the scene is handed back */
+    if ((scene->hole.deviceMap || scene->hole.deviceFluid) && !settle(h)) return false;
+    h->cpuMs = artfNowMs() - t0;
+    return true;
+}
+
+/* The queued frame done: its stage times from the events, its step counts from the pinned copy. */
+static bool settle(HipRenderer* h) {
+    if (!h->queued) return !h->lost;
+    h->queued = false;
+    if (!ok(h, artfHipStreamSync(h->stream), "frame")) return false;
+    const unsigned long long* slots = h->countersHost;
     unsigned long long counts[2] = {0, 0};
     for (int i = 0; i < ARTF_HIP_COUNTER_SLOTS; ++i) counts[0] += slots[i], counts[1] += slots[ARTF_HIP_COUNTER_SLOTS + i];
-
     ArtfHoleStats* st = &h->holeStats;
     st->traceMs = artfHipEventMs(h->events[EV_START], h->events[EV_TRACED]);
     st->resolveMs = artfHipEventMs(h->events[EV_TRACED], h->events[EV_RESOLVED]);
     st->bloomMs = artfHipEventMs(h->events[EV_RESOLVED], h->events[EV_BLOOMED]);
     st->toneMs = artfHipEventMs(h->events[EV_BLOOMED], h->events[EV_TONED]);
     st->uploadMs = 0;
-    (void)tUp;
-    st->rays = rays;
+    st->rays = h->frameRays;
     st->steps = counts[0];
     st->laneSlots = counts[1];
     st->bloomLevels = (int)h->frame.size[3];
-    h->drawn = true;
-    h->cpuMs = artfNowMs() - t0;
     return true;
 }
 
@@ -288,7 +314,7 @@ static void compose(const HipRenderer* h, uint32_t* dst) {
 
 static bool hipReadPixels(ArtfRenderer* r, uint32_t* dst) {
     HipRenderer* h = (HipRenderer*)r;
-    if (h->lost || !h->drawn) return false;
+    if (h->lost || !h->drawn || !settle(h)) return false;
     const size_t pixels = (size_t)h->width * h->height;
     if (!h->hole) {
         for (size_t i = 0; i < pixels; ++i) dst[i] = h->background;
@@ -301,7 +327,7 @@ static bool hipReadPixels(ArtfRenderer* r, uint32_t* dst) {
 
 static bool hipReadHdr(ArtfRenderer* r, float* red, float* green, float* blue) {
     HipRenderer* h = (HipRenderer*)r;
-    if (h->lost || !h->drawn || !h->hole) return false;
+    if (h->lost || !h->drawn || !h->hole || !settle(h)) return false;
     const size_t bytes = (size_t)h->width * h->height * sizeof(float);
     float* dst[3] = {red, green, blue};
     for (int c = 0; c < 3; ++c)
@@ -315,7 +341,8 @@ static uint32_t* hipPixels(ArtfRenderer* r) {
 }
 
 static void hipStats(const ArtfRenderer* r, ArtfFrameStats* out) {
-    const HipRenderer* h = (const HipRenderer*)r;
+    HipRenderer* h = (HipRenderer*)r; /* (settling the queued frame fills in what is read) */
+    settle(h);
     const ArtfHoleStats* st = &h->holeStats;
     out->cpuMs = h->cpuMs;
     out->lensMs = st->traceMs + st->resolveMs + st->bloomMs;
@@ -324,9 +351,10 @@ static void hipStats(const ArtfRenderer* r, ArtfFrameStats* out) {
 }
 
 static bool hipHoleStats(const ArtfRenderer* r, ArtfHoleStats* out) {
-    const HipRenderer* h = (const HipRenderer*)r;
+    HipRenderer* h = (HipRenderer*)r;
+    const bool settled = settle(h);
     *out = h->holeStats;
-    return h->drawn && h->hole;
+    return settled && h->drawn && h->hole;
 }
 
 static void hipSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* t) {
@@ -348,7 +376,7 @@ static void hipDestroy(ArtfRenderer* r) {
     for (int i = 0; i < EV_COUNT; ++i) artfHipEventDestroy(h->events[i]);
     artfHipStreamDestroy(h->stream);
     artfFree(h->overlay);
-    artfFree(h->countersHost);
+    artfHipHostFree(h->countersHost);
     artfFree(h);
 }
 
@@ -383,7 +411,13 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     artfHipDeviceInfo(o->device, name, sizeof name, &units);
     snprintf(h->description, sizeof h->description, "HIP, %s, %d compute units", name, units);
     h->tracing = artfHoleTracingDefaults();
-    h->countersHost = artfCalloc(2 * ARTF_HIP_COUNTER_SLOTS, sizeof *h->countersHost);
+    h->countersHost = artfHipHostMalloc(2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost);
+    if (!h->countersHost) {
+        artfLog("artful: HIP device %d: no pinned host memory", o->device);
+        hipDestroy(&h->base);
+        return NULL;
+    }
+    memset(h->countersHost, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost);
     hipResize(&h->base, o->width, o->height);
     return &h->base;
 }

@@ -26,6 +26,12 @@ void* artfHipMalloc(size_t bytes); /* NULL when the device has no room */
 void artfHipFree(void* p);
 int artfHipUpload(void* dst, const void* src, size_t bytes, void* stream);
 int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream);
+/* Device to host without waiting: dst must be pinned (artfHipHostMalloc) and is ready once the stream
+ * has been synchronised. */
+int artfHipDownloadAsync(void* dst, const void* src, size_t bytes, void* stream);
+/* Pinned host memory (NULL if none is left), and its release. */
+void* artfHipHostMalloc(size_t bytes);
+void artfHipHostFree(void* p);
 int artfHipMemset(void* dst, int value, size_t bytes, void* stream);
 /* Device to device. */
 int artfHipCopy(void* dst, const void* src, size_t bytes, void* stream);
