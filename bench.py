@@ -280,6 +280,7 @@ def animation(native, frames=30):
             hole_ms += (time.perf_counter() - t1) * 1e3
             if not r.draw(s):
                 raise RuntimeError("draw failed")
+        r.hole_stats()  # waits for the last frame
         wall = time.perf_counter() - t0
         last[where] = r.read_hdr()
         out[where] = {"setup_s": setup, "ms_per_frame": wall / frames * 1e3, "set_hole_ms_per_frame": hole_ms / frames,
