@@ -87,7 +87,14 @@ class HipTraceArgs(C.Structure):
     _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("map", C.c_void_p), ("fluid", C.c_void_p),
                 ("out", C.c_void_p * 3), ("counters", C.c_void_p), ("precision", C.c_int), ("maxSteps", C.c_int),
                 ("escapeRadius", C.c_float), ("disc", C.c_int), ("emissivity", C.c_float), ("jitter", C.c_uint32),
-                ("rowBegin", C.c_int), ("rowEnd", C.c_int)]
+                ("rowBegin", C.c_int), ("rowEnd", C.c_int), ("startStamp", C.c_void_p)]
+
+
+class HipFrameArgs(C.Structure):
+    _fields_ = [("trace", HipTraceArgs), ("hdr", C.c_void_p * 3), ("bloom", C.c_void_p * 3), ("bgra", C.c_void_p), ("report", C.c_void_p)]
+
+
+REPORT_WORDS = 7  # ARTF_HIP_REPORT_WORDS
 
 
 class HipAdjointArgs(C.Structure):

@@ -53,7 +53,7 @@ int artfHipDownloadAsync(void* dst, const void* src, size_t bytes, void* stream)
 
 void* artfHipHostMalloc(size_t bytes) {
     void* p = NULL;
-    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return NULL;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocCoherent) != hipSuccess) return NULL;
     return p;
 }
 
@@ -101,16 +101,39 @@ float artfHipEventMs(void* start, void* stop) {
 
 const char* artfHipErrorText(int status) { return hipGetErrorString((hipError_t)status); }
 
+/* The stream of a launch entry, with the thread's error state cleared: a launcher's status is
+ * hipGetLastError() after its launches, which must not report a call that failed before (an event
+ * never recorded, a refused copy) as this launch's. */
+static hipStream_t fresh(void* stream) {
+    (void)hipGetLastError();
+    return (hipStream_t)stream;
+}
+
 static bool frameFits(const ArtfHoleFrame* f) {
     return f && f->size[0] > 0 && f->size[1] > 0 && f->size[2] >= 1 && f->size[2] <= 8 && f->size[3] >= 1 && f->size[3] <= ARTF_BLOOM_LEVELS &&
            (uint64_t)f->size[0] * f->size[1] * f->size[2] * f->size[2] < (1ull << 31);
 }
 
+static bool traceFits(const ArtfHipTraceArgs* a) {
+    return a && frameFits(&a->frame) && a->map && a->fluid && a->out[0] && a->out[1] && a->out[2] && a->maxSteps >= 0 && a->maxSteps <= 1000000 &&
+           a->rowBegin >= 0;
+}
+
+/* The bloom level table artfHoleFrame builds: each level half the last (rounded up), packed. */
+static bool pyramidFits(const ArtfHoleFrame* f) {
+    uint32_t w = f->size[0], h = f->size[1], at = 0;
+    for (uint32_t k = 0; k < f->size[3]; ++k) {
+        w = (w + 1) / 2, h = (h + 1) / 2;
+        if (f->bloom[k][0] != at || f->bloom[k][1] != w || f->bloom[k][2] != h) return false;
+        at += w * h;
+    }
+    return true;
+}
+
 int artfHipTrace(const ArtfHipTraceArgs* a, void* stream) {
-    if (!a || !frameFits(&a->frame) || !a->map || !a->fluid || !a->out[0] || !a->out[1] || !a->out[2]) return (int)hipErrorInvalidValue;
-    if (a->maxSteps < 0 || a->maxSteps > 1000000 || a->rowBegin < 0) return (int)hipErrorInvalidValue;
-    if (a->precision == ARTF_PRECISION_FP32) return (int)launchTraceExact(*a, (hipStream_t)stream);
-    return (int)launchTraceFast(*a, (hipStream_t)stream);
+    if (!traceFits(a)) return (int)hipErrorInvalidValue;
+    if (a->precision == ARTF_PRECISION_FP32) return (int)launchTraceExact(*a, fresh(stream));
+    return (int)launchTraceFast(*a, fresh(stream));
 }
 
 int artfHipAdjointBlocks(const ArtfHoleFrame* f) { return frameFits(f) ? adjointBlocks(*f) : 0; }
@@ -124,30 +147,32 @@ int artfHipAdjoint(const ArtfHipAdjointArgs* args, void* stream) {
     if (a->disc != ARTF_DISC_ANALYTIC || (a->precision != ARTF_PRECISION_FP32 && a->precision != ARTF_PRECISION_FP64)) return (int)hipErrorInvalidValue;
     if (a->maxSteps < 0 || a->maxSteps > 2048 || a->frame.look[2] > 0.f || a->frame.boost[3] > 1.f || a->rowBegin != 0 || a->rowEnd != 0)
         return (int)hipErrorInvalidValue;
-    return (int)launchAdjoint(*args, (hipStream_t)stream);
+    return (int)launchAdjoint(*args, fresh(stream));
 }
 
 int artfHipResolve(const ArtfHoleFrame* f, float* const rays[3], float* const hdr[3], void* stream) {
     if (!frameFits(f) || !rays || !hdr) return (int)hipErrorInvalidValue;
-    return (int)launchResolve(*f, rays, hdr, (hipStream_t)stream);
+    return (int)launchResolve(*f, rays, hdr, fresh(stream));
 }
 
 int artfHipBloom(const ArtfHoleFrame* f, float* const hdr[3], float* const bloom[3], void* stream) {
-    if (!frameFits(f) || !hdr || !bloom) return (int)hipErrorInvalidValue;
-    /* the level table must be the one artfHoleFrame builds: each level half the last, packed */
-    uint32_t w = f->size[0], h = f->size[1], at = 0;
-    for (uint32_t k = 0; k < f->size[3]; ++k) {
-        w = (w + 1) / 2, h = (h + 1) / 2;
-        if (f->bloom[k][0] != at || f->bloom[k][1] != w || f->bloom[k][2] != h) return (int)hipErrorInvalidValue;
-        at += w * h;
-    }
-    return (int)launchBloom(*f, hdr, bloom, (hipStream_t)stream);
+    if (!frameFits(f) || !hdr || !bloom || !pyramidFits(f)) return (int)hipErrorInvalidValue;
+    return (int)launchBloom(*f, hdr, bloom, fresh(stream));
 }
 
 int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, void* stream) {
     if (!frameFits(f) || !hdr || !bloom || !bgra) return (int)hipErrorInvalidValue;
     if (f->bloom[0][1] != (f->size[0] + 1) / 2 || f->bloom[0][2] != (f->size[1] + 1) / 2) return (int)hipErrorInvalidValue;
-    return (int)launchToneMap(*f, hdr, bloom, bgra, (hipStream_t)stream);
+    return (int)launchToneMap(*f, hdr, bloom, bgra, fresh(stream));
+}
+
+int artfHipDrawFrame(const ArtfHipFrameArgs* a, void* stream) {
+    if (!a || !traceFits(&a->trace) || !pyramidFits(&a->trace.frame) || !a->bgra || (a->report && !a->trace.counters)) return (int)hipErrorInvalidValue;
+    const ArtfHoleFrame* f = &a->trace.frame;
+    if (a->trace.rowBegin != 0 || (a->trace.rowEnd != 0 && a->trace.rowEnd < (int)f->size[1])) return (int)hipErrorInvalidValue; /* whole frames */
+    for (int c = 0; c < 3; ++c)
+        if (!a->hdr[c] || !a->bloom[c] || (f->size[2] == 1 && a->trace.out[c] != a->hdr[c])) return (int)hipErrorInvalidValue;
+    return (int)launchFrame(*a, fresh(stream));
 }
 
 static bool batchFits(const ArtfHipBatchArgs* a) {
@@ -164,20 +189,20 @@ static bool batchFits(const ArtfHipBatchArgs* a) {
 
 int artfHipBatchReset(const ArtfHipBatchArgs* a, void* stream) {
     if (!batchFits(a)) return (int)hipErrorInvalidValue;
-    return (int)launchBatchReset(*a, (hipStream_t)stream);
+    return (int)launchBatchReset(*a, fresh(stream));
 }
 
 /* The views themselves live on the device, where the host cannot check them: the caller (the batch in
  * render/hole_batch.c) builds them from frames it has checked with frameFits and uploads them. */
 int artfHipBatchResetQueues(const ArtfHipBatchArgs* a, void* stream) {
     if (!batchFits(a)) return (int)hipErrorInvalidValue;
-    return (int)launchQueueReset(*a, (hipStream_t)stream);
+    return (int)launchQueueReset(*a, fresh(stream));
 }
 
 int artfHipTraceViews(const ArtfHipBatchArgs* a, void* stream) {
     if (!batchFits(a)) return (int)hipErrorInvalidValue;
-    if (a->precision == ARTF_PRECISION_FP32) return (int)launchViewsExact(*a, (hipStream_t)stream);
-    return (int)launchViewsFast(*a, (hipStream_t)stream);
+    if (a->precision == ARTF_PRECISION_FP32) return (int)launchViewsExact(*a, fresh(stream));
+    return (int)launchViewsFast(*a, fresh(stream));
 }
 
 int artfHipViewsOccupancy(int precision) {
@@ -222,28 +247,28 @@ static bool fluidFits(const ArtfHipFluid* f) {
 
 int artfHipFluidSeed(const ArtfHipFluid* f, void* stream) {
     if (!fluidFits(f)) return (int)hipErrorInvalidValue;
-    return (int)launchFluidSeed(*f, (hipStream_t)stream);
+    return (int)launchFluidSeed(*f, fresh(stream));
 }
 
 int artfHipFluidStir(const ArtfHipFluid* f, const ArtfFluidEddy* eddies, int count, void* stream) {
     if (!fluidFits(f) || !eddies || count < 0) return (int)hipErrorInvalidValue;
     if (count == 0) return 0;
-    return (int)launchFluidStir(*f, eddies, count, (hipStream_t)stream);
+    return (int)launchFluidStir(*f, eddies, count, fresh(stream));
 }
 
 int artfHipFluidStep(const ArtfHipFluid* f, const ArtfFluidEddy* eddies, void* stream) {
     if (!fluidFits(f) || !eddies) return (int)hipErrorInvalidValue;
-    return (int)launchFluidStep(*f, eddies, (hipStream_t)stream);
+    return (int)launchFluidStep(*f, eddies, fresh(stream));
 }
 
 int artfHipFluidField(const ArtfHipFluid* f, void* stream) {
     if (!fluidFits(f)) return (int)hipErrorInvalidValue;
-    return (int)launchFluidField(*f, (hipStream_t)stream);
+    return (int)launchFluidField(*f, fresh(stream));
 }
 
 int artfHipDiscPaint(float* map, void* stream) {
     if (!map) return (int)hipErrorInvalidValue;
-    return (int)launchDiscPaint(map, (hipStream_t)stream);
+    return (int)launchDiscPaint(map, fresh(stream));
 }
 
 } // extern "C"

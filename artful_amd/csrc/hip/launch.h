@@ -17,6 +17,17 @@ constexpr int kGroup = 256;
  * spills a little in the rarely taken FP64 paths and measures 5% faster (4 spills too much). */
 constexpr int wavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP64 || precision == ARTF_PRECISION_MIXED ? 3 : 4; }
 
+/* The GPU's clock (wall_clock64: artfHipWallClockHz ticks a second) stored to *at by the launch's first
+ * lane of its first workgroup as it starts, with a system-scope store (at may be host memory): a stage
+ * boundary read off the device's own clock, with no marker between kernels. NULL: nothing. */
+__device__ inline void stampStart(unsigned long long* at, unsigned long long* also = nullptr) {
+    if (at && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+        const unsigned long long now = (unsigned long long)wall_clock64();
+        __hip_atomic_store(at, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (also) __hip_atomic_store(also, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); /* the same instant, two boundaries */
+    }
+}
+
 /* FP32, the reference's arithmetic bit for bit (trace_exact.hip, contraction off). */
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
 /* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */
@@ -44,5 +55,7 @@ int adjointBlocks(const ArtfHoleFrame& f);
 hipError_t launchResolve(const ArtfHoleFrame& f, float* const rays[3], float* const hdr[3], hipStream_t stream);
 hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], hipStream_t stream);
 hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, hipStream_t stream);
+/* A whole frame: trace, resolve with level 0, the rest of the pyramid, tone map (post.hip). */
+hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream);
 
 } // namespace artf

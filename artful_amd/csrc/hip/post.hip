@@ -43,7 +43,8 @@ struct Level {
 
 /* Level 0: the mean of 2 x 2 pixels, each first held to what exceeds the threshold, at most the
  * ceiling (software.c:336-349). */
-__global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height) {
+__global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp) {
+    stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
     const int lastX = width - 1, lastY = height - 1;
@@ -58,31 +59,135 @@ __global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom
     }
 }
 
-/* Level k from k-1: four bilinear taps a texel either way, sixteen texels (software.c:350-364). */
-__global__ __launch_bounds__(256) void bloomDownKernel(Planes bloom, Level from, Level to) {
+/* A frame's resolve and level 0 in one pass: each thread resolves the 2 x 2 pixels under its level-0
+ * texel (resolveKernel's sums, each in ray order), then stores their HDR (each pixel by the one texel
+ * that holds it) and their level-0 mean, bloomFirstKernel's, from registers. Ray by ray, the twelve
+ * values it needs are loaded together (two pixels of a row as one 8-byte load where the row allows), and
+ * all loads come before the first store: a load issued after a store would wait for it. */
+__global__ __launch_bounds__(256) void resolveFirstKernel(Planes rays, Planes hdr, Planes bloom, Level to, int width, int height, int count,
+                                                          float share, unsigned long long* stamp) {
+    stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
-    const float cx = (float)x * 2.f + .5f, cy = (float)y * 2.f + .5f;
+    const int lastX = width - 1, lastY = height - 1, pixels = width * height;
+    const int ys[2] = {min(y * 2, lastY), min(y * 2 + 1, lastY)}, xs[2] = {min(x * 2, lastX), min(x * 2 + 1, lastX)};
+    const bool pair = (width & 1) == 0 && x * 2 + 1 <= lastX; /* both pixels there, 8-byte aligned */
+    float v[3][2][2] = {};
+    for (int k = 0; k < count; ++k) {
+        float got[3][2][2];
+        for (int c = 0; c < 3; ++c)
+            for (int dy = 0; dy < 2; ++dy) {
+                const float* row = rays.p[c] + (size_t)k * pixels + ys[dy] * width;
+                if (pair) {
+                    const float2 two = *(const float2*)(row + xs[0]);
+                    got[c][dy][0] = two.x, got[c][dy][1] = two.y;
+                } else {
+                    got[c][dy][0] = row[xs[0]], got[c][dy][1] = row[xs[1]];
+                }
+            }
+        for (int c = 0; c < 3; ++c)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) v[c][dy][dx] = v[c][dy][dx] + got[c][dy][dx];
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) v[c][dy][dx] = v[c][dy][dx] * share;
     for (int c = 0; c < 3; ++c) {
-        const float* level = bloom.p[c] + from.at;
-        float sum = 0;
-        for (int k = 0; k < 4; ++k) sum += levelTap(level, (int)from.w, (int)from.h, cx + (k & 1 ? 1.f : -1.f), cy + (k & 2 ? 1.f : -1.f));
-        bloom.p[c][to.at + (size_t)y * to.w + x] = sum * .25f;
+        for (int dy = 0; dy < 2; ++dy) {
+            float* row = hdr.p[c] + ys[dy] * width;
+            if (y * 2 + dy > lastY) continue;
+            if (pair) *(float2*)(row + xs[0]) = make_float2(v[c][dy][0], v[c][dy][1]);
+            else
+                for (int dx = 0; dx < 2; ++dx)
+                    if (x * 2 + dx <= lastX) row[xs[dx]] = v[c][dy][dx];
+        }
+#define BRIGHT(v) minRef(maxRef((v) - ARTF_BLOOM_THRESHOLD, 0.f), ARTF_BLOOM_CEILING)
+        bloom.p[c][to.at + (size_t)y * to.w + x] = (BRIGHT(v[c][0][0]) + BRIGHT(v[c][0][1]) + BRIGHT(v[c][1][0]) + BRIGHT(v[c][1][1])) * .25f;
+#undef BRIGHT
     }
 }
 
-/* Level k += the level above, spread by four taps half a texel either way (software.c:367-386). */
-__global__ __launch_bounds__(256) void bloomUpKernel(Planes bloom, Level from, Level to) {
+/* Level k's texel (x, y) from level k-1: four bilinear taps a texel either way, sixteen texels
+ * (software.c:350-364). level: the plane of level k-1, in memory or in LDS. */
+__device__ __forceinline__ float downTexel(const float* level, const Level& from, int x, int y) {
+    const float cx = (float)x * 2.f + .5f, cy = (float)y * 2.f + .5f;
+    float sum = 0;
+    for (int k = 0; k < 4; ++k) sum += levelTap(level, (int)from.w, (int)from.h, cx + (k & 1 ? 1.f : -1.f), cy + (k & 2 ? 1.f : -1.f));
+    return sum * .25f;
+}
+
+/* What level k's texel (x, y) gains from level k+1 on the way up: four taps half a texel either way
+ * (software.c:367-386). level: the plane of level k+1. */
+__device__ __forceinline__ float upGain(const float* level, const Level& from, int x, int y) {
+    const float sy = ((float)y + .5f) * .5f - .5f, sx = ((float)x + .5f) * .5f - .5f;
+    float sum = 0;
+    for (int k = 0; k < 4; ++k) sum += levelTap(level, (int)from.w, (int)from.h, sx + (k & 1 ? .5f : -.5f), sy + (k & 2 ? .5f : -.5f));
+    return sum * (.25f * ARTF_BLOOM_FADE);
+}
+
+__global__ __launch_bounds__(256) void bloomDownKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+    stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
-    const float sy = ((float)y + .5f) * .5f - .5f, sx = ((float)x + .5f) * .5f - .5f;
+    for (int c = 0; c < 3; ++c) bloom.p[c][to.at + (size_t)y * to.w + x] = downTexel(bloom.p[c] + from.at, from, x, y);
+}
+
+__global__ __launch_bounds__(256) void bloomUpKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+    stampStart(stamp);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= (int)to.w) return;
     for (int c = 0; c < 3; ++c) {
-        const float* level = bloom.p[c] + from.at;
-        float sum = 0;
-        for (int k = 0; k < 4; ++k) sum += levelTap(level, (int)from.w, (int)from.h, sx + (k & 1 ? .5f : -.5f), sy + (k & 2 ? .5f : -.5f));
         float* out = bloom.p[c] + to.at + (size_t)y * to.w + x;
-        *out += sum * (.25f * ARTF_BLOOM_FADE);
+        *out += upGain(bloom.p[c] + from.at, from, x, y);
     }
+}
+
+/* The small end of the pyramid in one workgroup: level `first` (as the launches before left it) is taken
+ * into LDS, levels first+1..last are built down and levels last-1..first+1 brought up there, with the
+ * arithmetic of bloomDownKernel and bloomUpKernel, and stored: the pyramid the launches level by level
+ * leave, in one launch instead of 2 (last - first) - 1. Levels first..last are packed one after
+ * another; together they hold at most kSmallFloats / 3 texels, and first+1..last at most kSmallWork
+ * (one compute unit does their work). */
+constexpr int kSmallThreads = 1024;
+constexpr int kSmallFloats = 32768; /* 128 KB of the compute unit's 160 KB */
+constexpr int kSmallWork = 4096;
+
+struct Pyramid {
+    Level level[ARTF_BLOOM_LEVELS];
+};
+
+__global__ __launch_bounds__(kSmallThreads) void bloomSmallKernel(Planes bloom, Pyramid p, int first, int last, unsigned long long* stamp) {
+    stampStart(stamp);
+    __shared__ float lds[kSmallFloats];
+    const int t = (int)threadIdx.x;
+    const uint32_t base = p.level[first].at;
+    const uint32_t n = p.level[last].at + p.level[last].w * p.level[last].h - base; /* texels a plane */
+    const Level& top = p.level[first];
+    for (int c = 0; c < 3; ++c)
+        for (uint32_t i = t; i < top.w * top.h; i += kSmallThreads) lds[c * n + i] = bloom.p[c][top.at + i];
+    __syncthreads();
+    for (int k = first + 1; k <= last; ++k) {
+        const Level from = p.level[k - 1], to = p.level[k];
+        for (uint32_t i = t; i < to.w * to.h; i += kSmallThreads) {
+            const int x = (int)(i % to.w), y = (int)(i / to.w);
+            for (int c = 0; c < 3; ++c) lds[c * n + (to.at - base) + i] = downTexel(lds + c * n + (from.at - base), from, x, y);
+        }
+        __syncthreads();
+    }
+    for (int k = last - 1; k > first; --k) {
+        const Level from = p.level[k + 1], to = p.level[k];
+        for (uint32_t i = t; i < to.w * to.h; i += kSmallThreads) {
+            const int x = (int)(i % to.w), y = (int)(i / to.w);
+            for (int c = 0; c < 3; ++c) {
+                float* out = lds + c * n + (to.at - base) + i;
+                *out += upGain(lds + c * n + (from.at - base), from, x, y);
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t skip = top.w * top.h; /* level `first` is as it came */
+    for (int c = 0; c < 3; ++c)
+        for (uint32_t i = skip + t; i < n; i += kSmallThreads) bloom.p[c][base + i] = lds[c * n + i];
 }
 
 /* Narkowicz's fit to ACES, then a square root (light to what a screen shows; lens.inc:387-391). */
@@ -93,9 +198,39 @@ __device__ __forceinline__ float toneCurve(float x) {
     return __builtin_sqrtf(v < 1.f ? v : 1.f);
 }
 
+/* The trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, written before this kernel on its
+ * stream) summed into settled[0..1] in host memory, with system-scope stores the host sees once the
+ * stream is synchronised, and cleared for the next frame's trace. One workgroup of 256 calls it. */
+__device__ void settleCounters(unsigned long long* counters, unsigned long long* settled) {
+    __shared__ unsigned long long part[2][4];
+    const int t = (int)threadIdx.x;
+    for (int half = 0; half < 2; ++half) {
+        unsigned long long sum = 0;
+        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += 256) {
+            sum += counters[half * ARTF_HIP_COUNTER_SLOTS + i];
+            counters[half * ARTF_HIP_COUNTER_SLOTS + i] = 0ull;
+        }
+        for (int lanes = 32; lanes > 0; lanes >>= 1) sum += __shfl_xor(sum, lanes);
+        if ((t & 63) == 0) part[half][t >> 6] = sum;
+    }
+    __syncthreads();
+    if (t < 2) __hip_atomic_store(settled + t, part[t][0] + part[t][1] + part[t][2] + part[t][3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 /* HDR plus the bloom's level 0 (read at half resolution, bilinear, 90% grey), times the exposure, tone
  * mapped, dithered by a hash of the pixel and truncated to bytes: BGRA, alpha 255 (lens.inc:395-440). */
-__global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread) {
+struct ToneReport {
+    unsigned long long* counters; /* the trace's, summed into settled[0..1] and cleared */
+    unsigned long long* settled;
+    unsigned long long* start;     /* the GPU's clock as the kernel starts */
+    unsigned long long* alsoStart; /* the same, a second stage's boundary */
+    unsigned long long* end;       /* the GPU's clock as the last workgroup (in launch order) ends */
+};
+
+__global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
+                                                  ToneReport report) {
+    stampStart(report.start, report.alsoStart);
+    if (report.settled && blockIdx.x == 0 && blockIdx.y == 0) settleCounters(report.counters, report.settled);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= width) return;
     const int bw = (int)l0.w, bh = (int)l0.h;
@@ -128,6 +263,9 @@ __global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint
         pixel |= byte << (16 - 8 * c);
     }
     out[at] = pixel;
+    /* thread 0 of the last workgroup always has a pixel (x < width), so it reaches here */
+    if (report.end && blockIdx.x == gridDim.x - 1 && blockIdx.y == gridDim.y - 1 && threadIdx.x == 0)
+        __hip_atomic_store(report.end, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 static Level levelOf(const ArtfHoleFrame& f, int k) {
@@ -142,29 +280,88 @@ hipError_t launchResolve(const ArtfHoleFrame& f, float* const rays[3], float* co
     return hipGetLastError();
 }
 
+static dim3 rowsOf(const Level& l) { return dim3((l.w + 255) / 256, l.h); }
+
+static uint32_t texelsFrom(const ArtfHoleFrame& f, int k) {
+    const int last = (int)f.size[3] - 1;
+    return f.bloom[last][0] + f.bloom[last][1] * f.bloom[last][2] - f.bloom[k][0];
+}
+
+/* The level bloomSmallKernel starts from: the first whose levels to the last fit its LDS and whose
+ * levels below it fit its work; the last level when none does (then it is not launched). */
+static int smallFirst(const ArtfHoleFrame& f) {
+    const int last = (int)f.size[3] - 1;
+    for (int k = 0; k < last; ++k)
+        if ((uint64_t)texelsFrom(f, k) * 3u <= (uint64_t)kSmallFloats && texelsFrom(f, k + 1) <= (uint32_t)kSmallWork) return k;
+    return last;
+}
+
+/* The pyramid above level 0, which is in place (software.c:419-426): levels 1..s down one launch each,
+ * the small end in one workgroup, and levels s..0 back up one launch each (s the small end's first
+ * level). stamp: given to the first launch; returns whether there was one. */
+static bool bloomRest(const ArtfHoleFrame& f, const Planes& b, hipStream_t stream, unsigned long long* stamp) {
+    const int last = (int)f.size[3] - 1, small = smallFirst(f);
+    bool any = false;
+    auto first = [&]() {
+        unsigned long long* at = any ? nullptr : stamp;
+        any = true;
+        return at;
+    };
+    for (int k = 1; k <= small; ++k) {
+        const Level to = levelOf(f, k);
+        hipLaunchKernelGGL(bloomDownKernel, rowsOf(to), dim3(256), 0, stream, b, levelOf(f, k - 1), to, first());
+    }
+    if (small < last) {
+        Pyramid p = {};
+        for (int k = 0; k <= last; ++k) p.level[k] = levelOf(f, k);
+        hipLaunchKernelGGL(bloomSmallKernel, dim3(1), dim3(kSmallThreads), 0, stream, b, p, small, last, first());
+    }
+    for (int k = small < last ? small : last - 1; k >= 0; --k) {
+        const Level to = levelOf(f, k);
+        hipLaunchKernelGGL(bloomUpKernel, rowsOf(to), dim3(256), 0, stream, b, levelOf(f, k + 1), to, first());
+    }
+    return any;
+}
+
 hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], hipStream_t stream) {
     const int levels = (int)f.size[3];
     if (levels < 1 || levels > ARTF_BLOOM_LEVELS) return hipErrorInvalidValue;
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
-    for (int k = 0; k < levels; ++k) {
-        const Level to = levelOf(f, k);
-        const dim3 grid((to.w + 255) / 256, to.h);
-        if (k == 0) hipLaunchKernelGGL(bloomFirstKernel, grid, dim3(256), 0, stream, h, b, to, (int)f.size[0], (int)f.size[1]);
-        else hipLaunchKernelGGL(bloomDownKernel, grid, dim3(256), 0, stream, b, levelOf(f, k - 1), to);
-    }
-    for (int k = levels - 2; k >= 0; --k) {
-        const Level to = levelOf(f, k);
-        hipLaunchKernelGGL(bloomUpKernel, dim3((to.w + 255) / 256, to.h), dim3(256), 0, stream, b, levelOf(f, k + 1), to);
-    }
+    const Level l0 = levelOf(f, 0);
+    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr);
+    bloomRest(f, b, stream, nullptr);
     return hipGetLastError();
 }
 
+static float spreadOf(const ArtfHoleFrame& f) { return f.look[3] * (1.f - ARTF_BLOOM_FADE); }
+
+static dim3 pixelRows(const ArtfHoleFrame& f) { return dim3((f.size[0] + 255) / 256, f.size[1]); }
+
 hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, hipStream_t stream) {
-    const int width = (int)f.size[0], height = (int)f.size[1];
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
-    const float spread = f.look[3] * (1.f - ARTF_BLOOM_FADE);
-    hipLaunchKernelGGL(toneKernel, dim3((unsigned)((width + 255) / 256), (unsigned)height), dim3(256), 0, stream, h, b, bgra, levelOf(f, 0), width,
-                       f.image[0], spread);
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, bgra, levelOf(f, 0), (int)f.size[0], f.image[0], spreadOf(f), ToneReport{});
+    return hipGetLastError();
+}
+
+hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
+    const ArtfHoleFrame& f = a.trace.frame;
+    unsigned long long* r = a.report;
+    ArtfHipTraceArgs t = a.trace;
+    t.startStamp = r ? r + 2 : nullptr;
+    const hipError_t traced = t.precision == ARTF_PRECISION_FP32 ? launchTraceExact(t, stream) : launchTraceFast(t, stream);
+    if (traced != hipSuccess) return traced;
+    const int side = (int)f.size[2];
+    Planes rays = {{t.out[0], t.out[1], t.out[2]}}, h = {{a.hdr[0], a.hdr[1], a.hdr[2]}}, b = {{a.bloom[0], a.bloom[1], a.bloom[2]}};
+    const Level l0 = levelOf(f, 0);
+    if (side > 1)
+        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(256), 0, stream, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
+                           r ? r + 3 : nullptr);
+    else
+        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr);
+    const bool rest = bloomRest(f, b, stream, r ? r + 4 : nullptr);
+    ToneReport tr = {};
+    if (r) tr = ToneReport{t.counters, r, r + 5, rest ? nullptr : r + 4, r + 6};
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), tr);
     return hipGetLastError();
 }
 

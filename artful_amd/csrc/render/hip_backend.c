@@ -3,10 +3,12 @@
  * The drop-in beside Artful 0.4.0's software and Vulkan backends (src/software/software.c:762-789,
  * src/vulkan/vulkan.c:1691-1783, 2566-2584): it fills the same vtable, reads the same scene data (the
  * map, uploaded when mapVersion changes; the fluid field, when fluidVersion changes; the frame's
- * uniforms from artfHoleFrame), and per frame runs trace -> (resolve) -> bloom -> tone map on one
- * stream, timed with events between the stages. It is plain C over include/artful_hip.h.
+ * uniforms from artfHoleFrame), and per frame runs trace -> resolve with bloom level 0 -> the rest of
+ * the bloom -> tone map on one stream (artfHipDrawFrame), its stages timed by the GPU's clock, which
+ * the kernels stamp as they start: nothing but the frame's kernels runs on the stream.
+ * It is plain C over include/artful_hip.h.
  *
- * A draw only queues its frame (its step counts copied to pinned memory by the stream itself); the
+ * A draw only queues its frame (its step counts summed into pinned memory by its last kernel); the
  * frame is waited for by what needs it: reading pixels or HDR back, the stats, the next draw's host
  * work never. So frames follow one another on the device without the host in between, as the Vulkan
  * backend's frames in flight do (vulkan.c vkDraw waits on a fence some frames old). A device fault in
@@ -24,13 +26,11 @@
 
 int artfHipCallsLeft = -1;
 
-enum { EV_START, EV_TRACED, EV_RESOLVED, EV_BLOOMED, EV_TONED, EV_COUNT };
-
 typedef struct HipRenderer {
     ArtfRenderer base;
     int device;
     void* stream;
-    void* events[EV_COUNT];
+    double clockHz; /* the GPU clock's ticks a second */
     char description[256];
     int width, height;
     ArtfHoleTracing tracing;
@@ -49,8 +49,8 @@ typedef struct HipRenderer {
     size_t bloomPixels;
     uint32_t* bgra;
     size_t bgraPixels;
-    unsigned long long* counters;     /* device: 2 * ARTF_HIP_COUNTER_SLOTS */
-    unsigned long long* countersHost; /* their copy (pinned), summed on the host once the frame is done */
+    unsigned long long* counters; /* device: 2 * ARTF_HIP_COUNTER_SLOTS, zero between frames */
+    unsigned long long* report;   /* pinned: the last frame's counts and clock stamps (ArtfHipFrameArgs) */
     /* the last frame */
     bool drawn, hole;
     bool queued; /* its stats still to be read off the device */
@@ -154,6 +154,10 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
     if (!h->counters) {
         h->counters = artfHipMalloc(2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long));
         if (!h->counters) return ok(h, 2, "allocate");
+        /* zero once: each frame's last kernel leaves them zero for the next */
+        if (!ok(h, artfHipMemset(h->counters, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long), h->stream), "clear") ||
+            !ok(h, artfHipStreamSync(h->stream), "clear"))
+            return false;
     }
     return true;
 }
@@ -227,38 +231,27 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     const int side = (int)h->frame.size[2];
     const size_t rays = (size_t)h->width * h->height * side * side;
     if (!ensure(h, bloomPixels, rays)) return false;
-    const double tUp = artfNowMs();
     if (!syncScene(h, scene)) return false;
 
-    ArtfHipTraceArgs a;
-    memset(&a, 0, sizeof a);
-    a.frame = h->frame;
-    if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a.frame64);
-    a.map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
-    a.fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
+    ArtfHipFrameArgs fa;
+    memset(&fa, 0, sizeof fa);
+    ArtfHipTraceArgs* a = &fa.trace;
+    a->frame = h->frame;
+    if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a->frame64);
+    a->map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
+    a->fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
     float** out = side == 1 ? h->hdr : h->rays;
-    for (int c = 0; c < 3; ++c) a.out[c] = out[c];
-    a.counters = h->counters;
-    a.precision = h->tracing.precision;
-    a.maxSteps = h->tracing.maxSteps;
-    a.escapeRadius = h->tracing.escapeRadius;
-    a.disc = h->tracing.disc;
-    a.emissivity = h->tracing.emissivity;
-    a.jitter = h->tracing.jitter;
-
-    if (!ok(h, artfHipMemset(h->counters, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long), h->stream), "clear")) return false;
-    if (!ok(h, artfHipEventRecord(h->events[EV_START], h->stream), "event")) return false;
-    if (!ok(h, artfHipTrace(&a, h->stream), "trace")) return false;
-    if (!ok(h, artfHipEventRecord(h->events[EV_TRACED], h->stream), "event")) return false;
-    if (side > 1 && !ok(h, artfHipResolve(&h->frame, h->rays, h->hdr, h->stream), "resolve")) return false;
-    if (!ok(h, artfHipEventRecord(h->events[EV_RESOLVED], h->stream), "event")) return false;
-    if (!ok(h, artfHipBloom(&h->frame, h->hdr, h->bloom, h->stream), "bloom")) return false;
-    if (!ok(h, artfHipEventRecord(h->events[EV_BLOOMED], h->stream), "event")) return false;
-    if (!ok(h, artfHipToneMap(&h->frame, h->hdr, h->bloom, h->bgra, h->stream), "tone map")) return false;
-    if (!ok(h, artfHipEventRecord(h->events[EV_TONED], h->stream), "event")) return false;
-    if (!ok(h, artfHipDownloadAsync(h->countersHost, h->counters, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost, h->stream), "queue"))
-        return false;
-    (void)tUp;
+    for (int c = 0; c < 3; ++c) a->out[c] = out[c], fa.hdr[c] = h->hdr[c], fa.bloom[c] = h->bloom[c];
+    a->counters = h->counters;
+    a->precision = h->tracing.precision;
+    a->maxSteps = h->tracing.maxSteps;
+    a->escapeRadius = h->tracing.escapeRadius;
+    a->disc = h->tracing.disc;
+    a->emissivity = h->tracing.emissivity;
+    a->jitter = h->tracing.jitter;
+    fa.bgra = h->bgra;
+    fa.report = h->report;
+    if (!ok(h, artfHipDrawFrame(&fa, h->stream), "frame")) return false;
     h->frameRays = rays;
     h->queued = true;
     h->drawn = true;
@@ -274,18 +267,20 @@ static bool settle(HipRenderer* h) {
     if (!h->queued) return !h->lost;
     h->queued = false;
     if (!ok(h, artfHipStreamSync(h->stream), "frame")) return false;
-    const unsigned long long* slots = h->countersHost;
-    unsigned long long counts[2] = {0, 0};
-    for (int i = 0; i < ARTF_HIP_COUNTER_SLOTS; ++i) counts[0] += slots[i], counts[1] += slots[ARTF_HIP_COUNTER_SLOTS + i];
     ArtfHoleStats* st = &h->holeStats;
-    st->traceMs = artfHipEventMs(h->events[EV_START], h->events[EV_TRACED]);
-    st->resolveMs = artfHipEventMs(h->events[EV_TRACED], h->events[EV_RESOLVED]);
-    st->bloomMs = artfHipEventMs(h->events[EV_RESOLVED], h->events[EV_BLOOMED]);
-    st->toneMs = artfHipEventMs(h->events[EV_BLOOMED], h->events[EV_TONED]);
+    /* each stage from its kernel's start to the next one's (the tone map: to its last workgroup's end);
+     * with several rays a pixel the resolve and bloom level 0 are one pass, reported as the resolve */
+    const unsigned long long* r = h->report;
+    const double ms = 1e3 / h->clockHz;
+    const bool resolved = h->frame.size[2] > 1;
+    st->traceMs = (double)(r[3] - r[2]) * ms;
+    st->resolveMs = resolved ? (double)(r[4] - r[3]) * ms : 0;
+    st->bloomMs = (double)(r[5] - r[resolved ? 4 : 3]) * ms;
+    st->toneMs = (double)(r[6] - r[5]) * ms;
     st->uploadMs = 0;
     st->rays = h->frameRays;
-    st->steps = counts[0];
-    st->laneSlots = counts[1];
+    st->steps = r[0];
+    st->laneSlots = r[1];
     st->bloomLevels = (int)h->frame.size[3];
     return true;
 }
@@ -373,10 +368,9 @@ static void hipDestroy(ArtfRenderer* r) {
     artfHipFree(h->counters);
     artfHipFree(h->map);
     artfHipFree(h->fluid);
-    for (int i = 0; i < EV_COUNT; ++i) artfHipEventDestroy(h->events[i]);
     artfHipStreamDestroy(h->stream);
     artfFree(h->overlay);
-    artfHipHostFree(h->countersHost);
+    artfHipHostFree(h->report);
     artfFree(h);
 }
 
@@ -398,11 +392,9 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     h->base.backend = ARTF_BACKEND_HIP;
     h->device = o->device;
     h->stream = artfHipStreamCreate();
-    for (int i = 0; i < EV_COUNT; ++i) h->events[i] = artfHipEventCreate();
-    bool ready = h->stream != NULL;
-    for (int i = 0; i < EV_COUNT; ++i) ready = ready && h->events[i];
-    if (!ready) {
-        artfLog("artful: HIP device %d: no stream or events", o->device);
+    h->clockHz = artfHipWallClockHz(o->device);
+    if (!h->stream || !(h->clockHz > 0)) {
+        artfLog("artful: HIP device %d: no stream or clock", o->device);
         hipDestroy(&h->base);
         return NULL;
     }
@@ -411,13 +403,13 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     artfHipDeviceInfo(o->device, name, sizeof name, &units);
     snprintf(h->description, sizeof h->description, "HIP, %s, %d compute units", name, units);
     h->tracing = artfHoleTracingDefaults();
-    h->countersHost = artfHipHostMalloc(2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost);
-    if (!h->countersHost) {
+    h->report = artfHipHostMalloc(ARTF_HIP_REPORT_WORDS * sizeof *h->report);
+    if (!h->report) {
         artfLog("artful: HIP device %d: no pinned host memory", o->device);
         hipDestroy(&h->base);
         return NULL;
     }
-    memset(h->countersHost, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof *h->countersHost);
+    memset(h->report, 0, ARTF_HIP_REPORT_WORDS * sizeof *h->report);
     hipResize(&h->base, o->width, o->height);
     return &h->base;
 }

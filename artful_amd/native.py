@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
         "artfHipStreamCreate": ([], P), "artfHipStreamDestroy": ([P], None), "artfHipStreamSync": ([P], I),
         "artfHipEventCreate": ([], P), "artfHipEventDestroy": ([P], None), "artfHipEventRecord": ([P, P], I),
         "artfHipEventMs": ([P, P], F), "artfHipErrorText": ([I], C.c_char_p),
+        "artfHipHostMalloc": ([C.c_size_t], P), "artfHipHostFree": ([P], None),
+        "artfHipDownloadAsync": ([P, P, C.c_size_t, P], I),
+        "artfHipDrawFrame": ([C.POINTER(abi.HipFrameArgs), P], I),
         "artfHipTrace": ([C.POINTER(abi.HipTraceArgs), P], I),
         "artfHipResolve": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),
         "artfHipBloom": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),

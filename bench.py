@@ -73,6 +73,54 @@ def time_frames(native, renderer, scene, precision, steps, warmup):
             "steps_per_frame": nsteps / staged, "active_lanes": nsteps / max(slots, 1), "bloom_levels": st.bloomLevels}
 
 
+def kernel_ms(native, scene, precision, launches=20, warmup=100):
+    """The trace kernel alone on the workload's frame: artfHipTrace launched back to back on a stream of
+    its own, bracketed by two HIP events on that stream; milliseconds a launch. The kernel writes the
+    rays' planes and counts its steps, as in a frame. The untimed launches first (about 0.1 s) bring the
+    GPU's clock up from where an idle spell left it."""
+    import ctypes as C
+    from artful_amd import abi
+    L = native.lib()
+    f, _ = scene.frame(WIDTH, HEIGHT)
+    a = abi.HipTraceArgs()
+    a.frame = f
+    a.frame64 = scene.frame64(WIDTH, HEIGHT)
+    a.precision = abi.PRECISION[precision]
+    host = [np.ascontiguousarray(scene.map()), np.ascontiguousarray(scene.fluid()[0])]
+    rays = WIDTH * HEIGHT * f.size[2] ** 2
+    counters = 2 * 1024 * 8  # 2 x ARTF_HIP_COUNTER_SLOTS words
+    held = [L.artfHipMalloc(h.nbytes) for h in host] + [L.artfHipMalloc(rays * 4) for _ in range(3)] + [L.artfHipMalloc(counters)]
+    stream, start, stop = L.artfHipStreamCreate(), L.artfHipEventCreate(), L.artfHipEventCreate()
+    try:
+        if not all(held) or not (stream and start and stop):
+            raise RuntimeError("no room for the kernel timing")
+        for d, h in zip(held, host):
+            if L.artfHipUpload(d, h.ctypes.data, h.nbytes, stream):
+                raise RuntimeError("upload failed")
+        a.map, a.fluid = held[0], held[1]
+        for c in range(3):
+            a.out[c] = held[2 + c]
+        a.counters = held[5]
+        if L.artfHipMemset(a.counters, 0, counters, stream):
+            raise RuntimeError("memset failed")
+        for _ in range(warmup):
+            if L.artfHipTrace(C.byref(a), stream):
+                raise RuntimeError("trace failed")
+        if L.artfHipStreamSync(stream) or L.artfHipEventRecord(start, stream):
+            raise RuntimeError("event failed")
+        for _ in range(launches):
+            if L.artfHipTrace(C.byref(a), stream):
+                raise RuntimeError("trace failed")
+        if L.artfHipEventRecord(stop, stream) or L.artfHipStreamSync(stream):
+            raise RuntimeError("trace failed")
+        return L.artfHipEventMs(start, stop) / launches
+    finally:
+        for d in held:
+            if d:
+                L.artfHipFree(d)
+        L.artfHipEventDestroy(start), L.artfHipEventDestroy(stop), L.artfHipStreamDestroy(stream)
+
+
 def rel_error(a, ref, floor=1e-3):
     rel = np.abs(a.astype(np.float64) - ref) / np.maximum(np.abs(ref), floor)
     return {"max": float(rel.max()), "mean": float(rel.mean()), "share_over_1e-3": float((rel > 1e-3).mean())}
@@ -369,8 +417,8 @@ def precision_study(native, width=3840, height=2160):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50, help="untimed frames first (50: ~60 ms, the GPU's clock up to speed)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
@@ -427,8 +475,11 @@ def main():
     ref64 = hdrs["fp64"].astype(np.float64)
     errors = {p: rel_error(h, ref64) for p, h in (("fp32", hdr32), ("fp32_fast", hdrs["fp32_fast"]), ("mixed", hdrs["mixed"]))}
 
+    # the dominant kernel alone, timed with HIP events on its stream: the roofline's and the tracers' rates
+    kernel = {p: kernel_ms(native, scene, p) for p in ("fp32", "fp32_fast", "mixed", "fp64")}
+    log(f"kernel ms: {kernel}")
     flops = fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0]
-    achieved = flops / (fp32["trace_ms"] * 1e-3) / 1e12
+    achieved = flops / (kernel["fp32"] * 1e-3) / 1e12
     traffic = traffic_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(native)
     adjoint = adjoint_fit(native)
@@ -451,10 +502,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic},
         "cpu_baseline": cpu,
-        "trace_rays_per_s": {"fp32": rays / (fp32["trace_ms"] * 1e-3),
-                             **{p: rays / (o["trace_ms"] * 1e-3) for p, o in others.items()}},
-        "fp32_over_fp64": others["fp64"]["trace_ms"] / fp32["trace_ms"],
-        "trace_ms": fp32["trace_ms"], "post_ms": fp32["post_ms"],
+        "trace_rays_per_s": {p: rays / (ms * 1e-3) for p, ms in kernel.items()},
+        "fp32_over_fp64": kernel["fp64"] / kernel["fp32"],
+        "trace_kernel_ms": kernel,
+        "frame_stages_ms": {"trace": fp32["trace_ms"], "post": fp32["post_ms"]},
         "steps_per_ray": fp32["steps_per_frame"] / rays, "active_lane_fraction": fp32["active_lanes"],
         "rel_error_vs_fp64": errors,
         "adjoint_over_forward": adjoint["adjoint_over_forward"],

@@ -29,7 +29,8 @@ int artfHipDownload(void* dst, const void* src, size_t bytes, void* stream);
 /* Device to host without waiting: dst must be pinned (artfHipHostMalloc) and is ready once the stream
  * has been synchronised. */
 int artfHipDownloadAsync(void* dst, const void* src, size_t bytes, void* stream);
-/* Pinned host memory (NULL if none is left), and its release. */
+/* Pinned host memory, coherent with the device (NULL if none is left), and its release: what a copy or a
+ * kernel's system-scope store puts there is seen by the host once the stream has been synchronised. */
 void* artfHipHostMalloc(size_t bytes);
 void artfHipHostFree(void* p);
 int artfHipMemset(void* dst, int value, size_t bytes, void* stream);
@@ -74,6 +75,9 @@ typedef struct ArtfHipTraceArgs {
     float emissivity;   /* the analytic disc's q */
     uint32_t jitter;    /* 0: the stratified grid; else a seed */
     int rowBegin, rowEnd; /* rows [rowBegin, rowEnd) of the image; rowEnd 0: to the last row */
+    /* NULL, or a word (device or artfHipHostMalloc memory) the launch's first workgroup sets to the GPU's
+     * clock (artfHipWallClockHz) as it starts */
+    unsigned long long* startStamp;
 } ArtfHipTraceArgs;
 
 int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
@@ -222,6 +226,32 @@ int artfHipAdjoint(const ArtfHipAdjointArgs* args, void* stream);
 int artfHipBloom(const ArtfHoleFrame* frame, float* const hdr[3], float* const bloom[3], void* stream);
 /* HDR + level 0 of the bloom to BGRA8 (alpha 255), width * height words. */
 int artfHipToneMap(const ArtfHoleFrame* frame, float* const hdr[3], float* const bloom[3], uint32_t* bgra, void* stream);
+
+/* One frame of the hole, as the reference's drawHole and lensFill make it (src/software/software.c:409-427,
+ * lens.inc:364-440): artfHipTrace, then artfHipResolve, artfHipBloom and artfHipToneMap, with the same
+ * bytes out, in fewer launches and with nothing else on the stream: the resolve and bloom level 0 are
+ * one pass (the rays' planes read once), the pyramid's small levels one workgroup's, and the frame's
+ * last kernel sums the trace's counters into the report and clears them (no fill, no copy), while
+ * each stage's kernel stamps the GPU's clock into the report (no events between kernels). */
+#define ARTF_HIP_REPORT_WORDS 7
+typedef struct ArtfHipFrameArgs {
+    /* out[]: the rays' planes (side > 1) or the HDR planes (side 1); whole frames (rowBegin 0, rowEnd 0
+     * or the height); startStamp: set by artfHipDrawFrame. counters: required with a report, and zero
+     * before the first frame (each frame leaves them zero). */
+    ArtfHipTraceArgs trace;
+    float* hdr[3];   /* device, width * height each */
+    float* bloom[3]; /* device, the pyramid as artfHipBloom lays it out */
+    uint32_t* bgra;  /* device, width * height words */
+    /* NULL, or host memory from artfHipHostMalloc, ARTF_HIP_REPORT_WORDS words, there once the stream is
+     * synchronised: [0] the frame's steps and [1] lane-slots (the sums of the counters' halves); the GPU's
+     * clock when [2] the trace started, [3] the resolve with level 0 started (with one ray a pixel: level
+     * 0), [4] the rest of the bloom started (with no more levels: the tone map), [5] the tone map started,
+     * [6] the tone map's last workgroup ended. Frames queued one after another share it: it holds the
+     * last one's. */
+    unsigned long long* report;
+} ArtfHipFrameArgs;
+
+int artfHipDrawFrame(const ArtfHipFrameArgs* args, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.BlackHoleDesc) == 92
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
-    assert C.sizeof(abi.HipTraceArgs) == 800
+    assert C.sizeof(abi.HipTraceArgs) == 808 and C.sizeof(abi.HipFrameArgs) == 872
     assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 96 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
     assert abi.MAP_FLOATS == 2887424
 
@@ -63,6 +63,39 @@ def test_trace_rejects_bad_arguments(native):
     P3 = C.c_void_p * 3
     assert L.artfHipBloom(C.byref(f), P3(1, 1, 1), P3(1, 1, 1), None) != 0  # level table not artfHoleFrame's
     assert L.artfHipToneMap(C.byref(f), P3(1, 1, 1), P3(1, 1, 1), None, None) != 0
+
+
+def test_frame_rejects_bad_arguments(native):
+    """artfHipDrawFrame checks what the four calls it stands for check, and its own: the output planes,
+    the HDR as the trace's output with one ray a pixel, counters under a report, whole frames only."""
+    L = native.lib()
+    INVALID = 1  # hipErrorInvalidValue: refused on the host, nothing launched
+    assert L.artfHipDrawFrame(None, None) == INVALID
+    a = abi.HipFrameArgs()
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID  # empty frame
+    f = abi.HoleFrame()
+    w, h = 16, 16
+    f.size[0], f.size[1], f.size[2], f.size[3] = w, h, 2, 1
+    f.bloom[0][0], f.bloom[0][1], f.bloom[0][2] = 0, 8, 8
+    a.trace.frame = f
+    a.trace.map = a.trace.fluid = 64
+    for c in range(3):
+        a.trace.out[c], a.hdr[c], a.bloom[c] = 64 * (c + 1), 64 * (c + 4), 64 * (c + 7)
+    a.bgra = 64
+    a.report = 64  # a report without counters
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
+    a.report = None
+    a.trace.rowBegin = 4  # a band, not a frame
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
+    a.trace.rowBegin = 0
+    a.bgra = None
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
+    a.bgra = 64
+    a.trace.frame.bloom[0][1] = 7  # a level table not artfHoleFrame's
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
+    a.trace.frame.bloom[0][1] = 8
+    a.trace.frame.size[2] = 1  # one ray a pixel: the trace must write the HDR planes
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
 
 
 def test_batch_rejects_bad_arguments(native):

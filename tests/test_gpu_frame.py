@@ -1,0 +1,165 @@
+"""One frame in one call (artfHipDrawFrame) and the bloom pyramid, against the oracle, bit for bit.
+
+artfHipDrawFrame stands for artfHipTrace, artfHipResolve, artfHipBloom and artfHipToneMap in a row (the
+reference's drawHole and lensFill, src/software/software.c:409-427, lens.inc:364-440): its HDR, its
+bloom pyramid and its BGRA must equal the oracle's for the same frame, and the step counts it leaves in
+host memory must be the sums of the counters artfHipTrace fills for that frame, which it leaves zero.
+The pyramid's small end is built by one workgroup in LDS: the sizes below put its first level at 0,
+1, 2, 3 and 4, and leave no two levels small enough together, or one level alone."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from artful_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+SLOTS = 1024  # ARTF_HIP_COUNTER_SLOTS
+
+
+class Device:
+    """Device buffers for one test, freed together."""
+
+    def __init__(self, L):
+        self.L, self.held = L, []
+
+    def alloc(self, nbytes):
+        p = self.L.artfHipMalloc(max(int(nbytes), 4))
+        assert p, "device memory"
+        self.held.append(p)
+        return p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        assert self.L.artfHipUpload(p, a.ctypes.data, a.nbytes, None) == 0
+        return p
+
+    def planes(self, n, value=None):
+        if value is None:
+            return [self.alloc(n * 4) for _ in range(3)]
+        return [self.upload(np.ascontiguousarray(value[c], np.float32)) for c in range(3)]
+
+    def read(self, p, n, dtype=np.float32):
+        out = np.zeros(n, dtype)
+        assert self.L.artfHipDownload(out.ctypes.data, p, out.nbytes, None) == 0
+        return out
+
+    def free(self):
+        for p in self.held:
+            self.L.artfHipFree(p)
+
+
+@pytest.fixture
+def dev(gpu):
+    d = Device(gpu.lib())
+    yield d
+    d.free()
+
+
+def small_first(f):
+    """Where the one-workgroup pass starts: the first level k whose levels k..last fit 32768 floats of
+    LDS and whose levels k+1..last hold at most 4096 texels; the last level when none does."""
+    last = f.size[3] - 1
+    end = f.bloom[last][0] + f.bloom[last][1] * f.bloom[last][2]
+    for k in range(last):
+        if (end - f.bloom[k][0]) * 3 <= 32768 and end - f.bloom[k + 1][0] <= 4096:
+            return k
+    return last
+
+
+PYRAMIDS = [(48, 27, 8, 0), (320, 200, 8, 1), (640, 400, 8, 2), (1920, 1080, 8, 3), (3840, 2160, 8, 4), (1920, 1080, 3, 2),
+            (1920, 1080, 2, 1), (1920, 1080, 1, 0), (333, 77, 8, 0), (1001, 3, 8, 0), (1, 1, 8, 0), (5, 3, 4, 0)]
+
+
+@pytest.mark.parametrize("w,h,levels,first", PYRAMIDS)
+def test_bloom_and_tone_map_equal_oracle(w, h, levels, first, gpu, dev, scene_of, oracle):
+    """artfHipBloom and artfHipToneMap on HDR with light on both sides of the bloom's threshold and past
+    its ceiling, against oracleBloom and oracleToneMap."""
+    L = gpu.lib()
+    f, bloom_pixels = scene_of("config1").frame(w, h, levels)
+    print(f"pyramid {w}x{h} levels {f.size[3]} small end from {small_first(f)}", flush=True)
+    if first is not None:
+        assert small_first(f) == first
+    rng = np.random.default_rng(w * 7 + h * 3 + levels)
+    hdr = (rng.exponential(2.0, (3, w * h)) * (rng.random((3, w * h)) < .3) * 30).astype(np.float32)
+    d_hdr, d_bloom = dev.planes(w * h, hdr), dev.planes(bloom_pixels)
+    d_bgra = dev.alloc(w * h * 4)
+    P3 = C.c_void_p * 3
+    assert L.artfHipBloom(C.byref(f), P3(*d_hdr), P3(*d_bloom), None) == 0
+    assert L.artfHipToneMap(C.byref(f), P3(*d_hdr), P3(*d_bloom), d_bgra, None) == 0
+    assert L.artfHipStreamSync(None) == 0
+    got = np.stack([dev.read(d_bloom[c], bloom_pixels) for c in range(3)])
+    want = oracle.bloom(f, hdr.reshape(3, h, w), bloom_pixels)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    bgra = dev.read(d_bgra, w * h, np.uint32).reshape(h, w)
+    np.testing.assert_array_equal(bgra, oracle.tone_map(f, hdr.reshape(3, h, w), want))
+
+
+FRAMES = [("config1", 320, 180), ("config2_small", 192, 108), ("moving_eye", 101, 67), ("inner_given", 16, 9), ("gpu_vs_cpu", 320, 200),
+          ("moving_eye", 1, 1)]
+
+
+def frame_args(L, dev, s, w, h, f, bloom_pixels):
+    side = f.size[2]
+    a = abi.HipFrameArgs()
+    a.trace.frame = f
+    a.trace.map, a.trace.fluid = dev.upload(s.map()), dev.upload(s.fluid()[0])
+    hdr = dev.planes(w * h)
+    out = hdr if side == 1 else dev.planes(w * h * side * side)
+    for c, p in enumerate(out):
+        a.trace.out[c] = p
+    for c, (ph, pb) in enumerate(zip(hdr, dev.planes(bloom_pixels))):
+        a.hdr[c], a.bloom[c] = ph, pb
+    a.bgra = dev.alloc(w * h * 4)
+    a.trace.counters = dev.upload(np.zeros(2 * SLOTS, np.uint64))
+    return a
+
+
+@pytest.mark.parametrize("name,w,h", FRAMES)
+def test_frame_equals_oracle(name, w, h, gpu, dev, scene_of, oracle):
+    """Two frames in a row through artfHipDrawFrame: HDR, pyramid and BGRA equal the oracle's render, the
+    reported counts equal what artfHipTrace counts for the frame, the counters are left zero, and the
+    stage stamps are in order."""
+    L = gpu.lib()
+    s = scene_of(name)
+    f, bloom_pixels = s.frame(w, h)
+    print(f"frame {name} {w}x{h} side {f.size[2]} levels {f.size[3]}", flush=True)
+    a = frame_args(L, dev, s, w, h, f, bloom_pixels)
+    report = L.artfHipHostMalloc(8 * abi.REPORT_WORDS)
+    try:
+        words = (C.c_uint64 * abi.REPORT_WORDS).from_address(report)
+        a.report = report
+        counts = []
+        for _ in range(2):
+            for i in range(abi.REPORT_WORDS):
+                words[i] = 0
+            assert L.artfHipDrawFrame(C.byref(a), None) == 0
+            assert L.artfHipStreamSync(None) == 0
+            counts.append((words[0], words[1]))
+            stamps = list(words[2:])
+            print("clock stamps", [x - stamps[0] for x in stamps])
+            assert 0 < stamps[0] <= stamps[1] <= stamps[2] <= stamps[3] <= stamps[4]
+        assert counts[0] == counts[1]
+        assert not dev.read(a.trace.counters, 2 * SLOTS, np.uint64).any(), "counters left zero"
+
+        hdr = np.stack([dev.read(a.hdr[c], w * h) for c in range(3)]).reshape(3, h, w)
+        pyramid = np.stack([dev.read(a.bloom[c], bloom_pixels) for c in range(3)])
+        bgra = dev.read(a.bgra, w * h, np.uint32).reshape(h, w)
+        want_hdr, want_bloom, want_bgra = oracle.render(f, s.map(), s.fluid()[0], bloom_pixels)
+        np.testing.assert_array_equal(hdr.view(np.uint32), want_hdr.view(np.uint32))
+        np.testing.assert_array_equal(pyramid.view(np.uint32), want_bloom.view(np.uint32))
+        np.testing.assert_array_equal(bgra, want_bgra)
+
+        # the same frame's counts from artfHipTrace alone
+        t = abi.HipTraceArgs.from_buffer_copy(a.trace)
+        assert L.artfHipTrace(C.byref(t), None) == 0
+        assert L.artfHipStreamSync(None) == 0
+        slots = dev.read(a.trace.counters, 2 * SLOTS, np.uint64)
+        assert (int(slots[:SLOTS].sum()), int(slots[SLOTS:].sum())) == counts[0]
+        assert 0 < counts[0][0] <= counts[0][1]
+    finally:
+        L.artfHipHostFree(report)

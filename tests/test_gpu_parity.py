@@ -174,7 +174,7 @@ def test_device_lost_contract(gpu, native, scene_of):
     s = scene_of("config1")
     r = gpu.Renderer(32, 18)
     assert r.draw(s)
-    native.calls_left(2)
+    native.calls_left(0)
     try:
         lost = not r.draw(s)
     finally:
