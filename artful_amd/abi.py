@@ -91,7 +91,8 @@ class HipTraceArgs(C.Structure):
 
 
 class HipFrameArgs(C.Structure):
-    _fields_ = [("trace", HipTraceArgs), ("hdr", C.c_void_p * 3), ("bloom", C.c_void_p * 3), ("bgra", C.c_void_p), ("report", C.c_void_p)]
+    _fields_ = [("trace", HipTraceArgs), ("hdr", C.c_void_p * 3), ("bloom", C.c_void_p * 3), ("bgra", C.c_void_p), ("report", C.c_void_p),
+                ("postStream", C.c_void_p), ("handoff", C.c_void_p), ("done", C.c_void_p)]
 
 
 REPORT_WORDS = 7  # ARTF_HIP_REPORT_WORDS

@@ -93,6 +93,24 @@ void artfHipEventDestroy(void* event) {
 
 int artfHipEventRecord(void* event, void* stream) { return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream); }
 
+void* artfHipEventCreateOrder(void) {
+    hipEvent_t e = NULL;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming | hipEventReleaseToDevice) != hipSuccess) return NULL;
+    return (void*)e;
+}
+
+int artfHipStreamWait(void* stream, void* event) { return (int)hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0); }
+
+int artfHipEventDone(void* event) {
+    const hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError(); /* not an error: leave nothing for the next launch's check */
+        return 0;
+    }
+    return -1;
+}
+
 float artfHipEventMs(void* start, void* stop) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop) != hipSuccess) return -1.f;
@@ -172,6 +190,7 @@ int artfHipDrawFrame(const ArtfHipFrameArgs* a, void* stream) {
     if (a->trace.rowBegin != 0 || (a->trace.rowEnd != 0 && a->trace.rowEnd < (int)f->size[1])) return (int)hipErrorInvalidValue; /* whole frames */
     for (int c = 0; c < 3; ++c)
         if (!a->hdr[c] || !a->bloom[c] || (f->size[2] == 1 && a->trace.out[c] != a->hdr[c])) return (int)hipErrorInvalidValue;
+    if (a->postStream && (!a->handoff || a->postStream == stream)) return (int)hipErrorInvalidValue;
     return (int)launchFrame(*a, fresh(stream));
 }
 

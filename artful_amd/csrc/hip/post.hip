@@ -41,10 +41,39 @@ struct Level {
     uint32_t at, w, h, pad;
 };
 
+/* The trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, written before this kernel on its
+ * stream) summed into settled[0..1] in host memory, with system-scope stores the host sees once the
+ * stream is synchronised, and cleared for the next frame's trace. The first workgroup (256 threads) of
+ * the kernel that follows the trace calls it, before anything can return. */
+struct Settle {
+    unsigned long long* counters;
+    unsigned long long* settled; /* NULL: nothing to do */
+};
+
+__device__ void settleCounters(const Settle& z) {
+    if (!z.settled || blockIdx.x != 0 || blockIdx.y != 0) return;
+    unsigned long long* counters = z.counters;
+    __shared__ unsigned long long part[2][4];
+    const int t = (int)threadIdx.x;
+    for (int half = 0; half < 2; ++half) {
+        unsigned long long sum = 0;
+        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += 256) {
+            sum += counters[half * ARTF_HIP_COUNTER_SLOTS + i];
+            counters[half * ARTF_HIP_COUNTER_SLOTS + i] = 0ull;
+        }
+        for (int lanes = 32; lanes > 0; lanes >>= 1) sum += __shfl_xor(sum, lanes);
+        if ((t & 63) == 0) part[half][t >> 6] = sum;
+    }
+    __syncthreads();
+    if (t < 2) __hip_atomic_store(z.settled + t, part[t][0] + part[t][1] + part[t][2] + part[t][3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 /* Level 0: the mean of 2 x 2 pixels, each first held to what exceeds the threshold, at most the
  * ceiling (software.c:336-349). */
-__global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp) {
+__global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp,
+                                                        Settle z) {
     stampStart(stamp);
+    settleCounters(z);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
     const int lastX = width - 1, lastY = height - 1;
@@ -65,8 +94,9 @@ __global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom
  * values it needs are loaded together (two pixels of a row as one 8-byte load where the row allows), and
  * all loads come before the first store: a load issued after a store would wait for it. */
 __global__ __launch_bounds__(256) void resolveFirstKernel(Planes rays, Planes hdr, Planes bloom, Level to, int width, int height, int count,
-                                                          float share, unsigned long long* stamp) {
+                                                          float share, unsigned long long* stamp, Settle z) {
     stampStart(stamp);
+    settleCounters(z);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
     const int lastX = width - 1, lastY = height - 1, pixels = width * height;
@@ -148,7 +178,9 @@ __global__ __launch_bounds__(256) void bloomUpKernel(Planes bloom, Level from, L
  * leave, in one launch instead of 2 (last - first) - 1. Levels first..last are packed one after
  * another; together they hold at most kSmallFloats / 3 texels, and first+1..last at most kSmallWork
  * (one compute unit does their work). */
-constexpr int kSmallThreads = 1024;
+/* 256 threads (a wave per SIMD): the workgroup finds room as soon as one trace workgroup ends, so it
+ * runs beside the next frame's trace; 1024 would wait for a whole compute unit to drain. */
+constexpr int kSmallThreads = 256;
 constexpr int kSmallFloats = 32768; /* 128 KB of the compute unit's 160 KB */
 constexpr int kSmallWork = 4096;
 
@@ -198,39 +230,18 @@ __device__ __forceinline__ float toneCurve(float x) {
     return __builtin_sqrtf(v < 1.f ? v : 1.f);
 }
 
-/* The trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, written before this kernel on its
- * stream) summed into settled[0..1] in host memory, with system-scope stores the host sees once the
- * stream is synchronised, and cleared for the next frame's trace. One workgroup of 256 calls it. */
-__device__ void settleCounters(unsigned long long* counters, unsigned long long* settled) {
-    __shared__ unsigned long long part[2][4];
-    const int t = (int)threadIdx.x;
-    for (int half = 0; half < 2; ++half) {
-        unsigned long long sum = 0;
-        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += 256) {
-            sum += counters[half * ARTF_HIP_COUNTER_SLOTS + i];
-            counters[half * ARTF_HIP_COUNTER_SLOTS + i] = 0ull;
-        }
-        for (int lanes = 32; lanes > 0; lanes >>= 1) sum += __shfl_xor(sum, lanes);
-        if ((t & 63) == 0) part[half][t >> 6] = sum;
-    }
-    __syncthreads();
-    if (t < 2) __hip_atomic_store(settled + t, part[t][0] + part[t][1] + part[t][2] + part[t][3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
+/* Where the tone map stamps the GPU's clock (each NULL for none). */
+struct ToneStamps {
+    unsigned long long* start;     /* as the kernel starts */
+    unsigned long long* alsoStart; /* the same instant, a second stage's boundary */
+    unsigned long long* end;       /* as the last workgroup (in launch order) ends */
+};
 
 /* HDR plus the bloom's level 0 (read at half resolution, bilinear, 90% grey), times the exposure, tone
  * mapped, dithered by a hash of the pixel and truncated to bytes: BGRA, alpha 255 (lens.inc:395-440). */
-struct ToneReport {
-    unsigned long long* counters; /* the trace's, summed into settled[0..1] and cleared */
-    unsigned long long* settled;
-    unsigned long long* start;     /* the GPU's clock as the kernel starts */
-    unsigned long long* alsoStart; /* the same, a second stage's boundary */
-    unsigned long long* end;       /* the GPU's clock as the last workgroup (in launch order) ends */
-};
-
 __global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
-                                                  ToneReport report) {
-    stampStart(report.start, report.alsoStart);
-    if (report.settled && blockIdx.x == 0 && blockIdx.y == 0) settleCounters(report.counters, report.settled);
+                                                  ToneStamps stamps) {
+    stampStart(stamps.start, stamps.alsoStart);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= width) return;
     const int bw = (int)l0.w, bh = (int)l0.h;
@@ -264,8 +275,8 @@ __global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint
     }
     out[at] = pixel;
     /* thread 0 of the last workgroup always has a pixel (x < width), so it reaches here */
-    if (report.end && blockIdx.x == gridDim.x - 1 && blockIdx.y == gridDim.y - 1 && threadIdx.x == 0)
-        __hip_atomic_store(report.end, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (stamps.end && blockIdx.x == gridDim.x - 1 && blockIdx.y == gridDim.y - 1 && threadIdx.x == 0)
+        __hip_atomic_store(stamps.end, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 static Level levelOf(const ArtfHoleFrame& f, int k) {
@@ -328,7 +339,8 @@ hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const
     if (levels < 1 || levels > ARTF_BLOOM_LEVELS) return hipErrorInvalidValue;
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
     const Level l0 = levelOf(f, 0);
-    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr);
+    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr,
+                       Settle{});
     bloomRest(f, b, stream, nullptr);
     return hipGetLastError();
 }
@@ -339,7 +351,7 @@ static dim3 pixelRows(const ArtfHoleFrame& f) { return dim3((f.size[0] + 255) / 
 
 hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, hipStream_t stream) {
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
-    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, bgra, levelOf(f, 0), (int)f.size[0], f.image[0], spreadOf(f), ToneReport{});
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, bgra, levelOf(f, 0), (int)f.size[0], f.image[0], spreadOf(f), ToneStamps{});
     return hipGetLastError();
 }
 
@@ -353,16 +365,26 @@ hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
     const int side = (int)f.size[2];
     Planes rays = {{t.out[0], t.out[1], t.out[2]}}, h = {{a.hdr[0], a.hdr[1], a.hdr[2]}}, b = {{a.bloom[0], a.bloom[1], a.bloom[2]}};
     const Level l0 = levelOf(f, 0);
+    const Settle z = {t.counters, r};
+    /* the rest on the second stream, once the trace is done: the first can take the next frame's trace */
+    hipStream_t post = stream;
+    hipError_t e;
+    if (a.postStream) {
+        post = (hipStream_t)a.postStream;
+        if ((e = hipEventRecord((hipEvent_t)a.handoff, stream)) != hipSuccess || (e = hipStreamWaitEvent(post, (hipEvent_t)a.handoff, 0)) != hipSuccess)
+            return e;
+    }
     if (side > 1)
-        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(256), 0, stream, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
-                           r ? r + 3 : nullptr);
+        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(256), 0, post, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
+                           r ? r + 3 : nullptr, z);
     else
-        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr);
-    const bool rest = bloomRest(f, b, stream, r ? r + 4 : nullptr);
-    ToneReport tr = {};
-    if (r) tr = ToneReport{t.counters, r, r + 5, rest ? nullptr : r + 4, r + 6};
-    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), tr);
-    return hipGetLastError();
+        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
+    const bool rest = bloomRest(f, b, post, r ? r + 4 : nullptr);
+    const ToneStamps ts = r ? ToneStamps{r + 5, rest ? nullptr : r + 4, r + 6} : ToneStamps{};
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, post, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), ts);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (a.postStream && a.done) return hipEventRecord((hipEvent_t)a.done, post);
+    return hipSuccess;
 }
 
 /* A batch's workspace to zero and its start stamped, by a kernel rather than a memset, so that a graph

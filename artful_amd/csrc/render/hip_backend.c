@@ -4,8 +4,11 @@
  * src/vulkan/vulkan.c:1691-1783, 2566-2584): it fills the same vtable, reads the same scene data (the
  * map, uploaded when mapVersion changes; the fluid field, when fluidVersion changes; the frame's
  * uniforms from artfHoleFrame), and per frame runs trace -> resolve with bloom level 0 -> the rest of
- * the bloom -> tone map on one stream (artfHipDrawFrame), its stages timed by the GPU's clock, which
- * the kernels stamp as they start: nothing but the frame's kernels runs on the stream.
+ * the bloom -> tone map (artfHipDrawFrame), its stages timed by the GPU's clock, which the kernels
+ * stamp as they start. The trace runs on one stream and the resolve, bloom and tone map on a second,
+ * so a frame's post passes run beside the next frame's trace; frames take turns with two sets of rays'
+ * planes, counters, HDR and pyramid, and a set is used again only after the frame before last is done
+ * with it (the trace's stream waits for its tone map's event when it has not ended yet).
  * It is plain C over include/artful_hip.h.
  *
  * A draw only queues its frame (its step counts summed into pinned memory by its last kernel); the
@@ -29,7 +32,11 @@ int artfHipCallsLeft = -1;
 typedef struct HipRenderer {
     ArtfRenderer base;
     int device;
-    void* stream;
+    void* stream;   /* the traces */
+    void* post;     /* the resolves, blooms and tone maps */
+    void* handoff[2], *done[2]; /* per set: its trace ended; its post passes ended */
+    bool busy[2];   /* per set: a frame's post passes may still use it */
+    int set;        /* the set of the last frame */
     double clockHz; /* the GPU clock's ticks a second */
     char description[256];
     int width, height;
@@ -41,15 +48,15 @@ typedef struct HipRenderer {
     float* fluid;
     uint64_t fluidScene, fluidVersion;
     /* per-size targets */
-    float* hdr[3];
+    float* hdr[2][3];
     size_t hdrPixels;
-    float* rays[3];
+    float* rays[2][3];
     size_t rayCount;
-    float* bloom[3];
+    float* bloom[2][3];
     size_t bloomPixels;
     uint32_t* bgra;
     size_t bgraPixels;
-    unsigned long long* counters; /* device: 2 * ARTF_HIP_COUNTER_SLOTS, zero between frames */
+    unsigned long long* counters[2]; /* device, per set: 2 * ARTF_HIP_COUNTER_SLOTS, zero between frames */
     unsigned long long* report;   /* pinned: the last frame's counts and clock stamps (ArtfHipFrameArgs) */
     /* the last frame */
     bool drawn, hole;
@@ -130,34 +137,34 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
     const bool replace = pixels != h->hdrPixels || (rays > pixels && rays > h->rayCount) || bloomPixels > h->bloomPixels;
     if (replace && h->queued && !settle(h)) return false;
     if (pixels != h->hdrPixels) {
-        freePlanes(h->hdr);
+        freePlanes(h->hdr[0]), freePlanes(h->hdr[1]);
         artfHipFree(h->bgra);
         h->bgra = NULL;
         h->hdrPixels = h->bgraPixels = 0;
-        if (!allocPlanes(h, h->hdr, pixels)) return false;
+        if (!allocPlanes(h, h->hdr[0], pixels) || !allocPlanes(h, h->hdr[1], pixels)) return false;
         h->bgra = artfHipMalloc(pixels * 4);
         if (!h->bgra) return ok(h, 2, "allocate");
         h->hdrPixels = h->bgraPixels = pixels;
     }
     if (rays > pixels && rays > h->rayCount) {
-        freePlanes(h->rays);
+        freePlanes(h->rays[0]), freePlanes(h->rays[1]);
         h->rayCount = 0;
-        if (!allocPlanes(h, h->rays, rays)) return false;
+        if (!allocPlanes(h, h->rays[0], rays) || !allocPlanes(h, h->rays[1], rays)) return false;
         h->rayCount = rays;
     }
     if (bloomPixels > h->bloomPixels) {
-        freePlanes(h->bloom);
+        freePlanes(h->bloom[0]), freePlanes(h->bloom[1]);
         h->bloomPixels = 0;
-        if (!allocPlanes(h, h->bloom, bloomPixels)) return false;
+        if (!allocPlanes(h, h->bloom[0], bloomPixels) || !allocPlanes(h, h->bloom[1], bloomPixels)) return false;
         h->bloomPixels = bloomPixels;
     }
-    if (!h->counters) {
-        h->counters = artfHipMalloc(2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long));
-        if (!h->counters) return ok(h, 2, "allocate");
-        /* zero once: each frame's last kernel leaves them zero for the next */
-        if (!ok(h, artfHipMemset(h->counters, 0, 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long), h->stream), "clear") ||
-            !ok(h, artfHipStreamSync(h->stream), "clear"))
-            return false;
+    for (int k = 0; k < 2; ++k) {
+        if (h->counters[k]) continue;
+        const size_t bytes = 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long);
+        h->counters[k] = artfHipMalloc(bytes);
+        if (!h->counters[k]) return ok(h, 2, "allocate");
+        /* zero once: each frame's resolve leaves them zero for the set's next frame */
+        if (!ok(h, artfHipMemset(h->counters[k], 0, bytes, h->stream), "clear") || !ok(h, artfHipStreamSync(h->stream), "clear")) return false;
     }
     return true;
 }
@@ -240,9 +247,16 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a->frame64);
     a->map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
     a->fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
-    float** out = side == 1 ? h->hdr : h->rays;
-    for (int c = 0; c < 3; ++c) a->out[c] = out[c], fa.hdr[c] = h->hdr[c], fa.bloom[c] = h->bloom[c];
-    a->counters = h->counters;
+    /* the other set than the last frame's, once the frame before last is done with it */
+    const int set = h->set ^ 1;
+    if (h->busy[set]) {
+        const int done = artfHipEventDone(h->done[set]);
+        if (done < 0) return ok(h, 1, "frame order");
+        if (!done && !ok(h, artfHipStreamWait(h->stream, h->done[set]), "frame order")) return false;
+    }
+    float** out = side == 1 ? h->hdr[set] : h->rays[set];
+    for (int c = 0; c < 3; ++c) a->out[c] = out[c], fa.hdr[c] = h->hdr[set][c], fa.bloom[c] = h->bloom[set][c];
+    a->counters = h->counters[set];
     a->precision = h->tracing.precision;
     a->maxSteps = h->tracing.maxSteps;
     a->escapeRadius = h->tracing.escapeRadius;
@@ -251,7 +265,12 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     a->jitter = h->tracing.jitter;
     fa.bgra = h->bgra;
     fa.report = h->report;
+    fa.postStream = h->post;
+    fa.handoff = h->handoff[set];
+    fa.done = h->done[set];
     if (!ok(h, artfHipDrawFrame(&fa, h->stream), "frame")) return false;
+    h->busy[set] = true;
+    h->set = set;
     h->frameRays = rays;
     h->queued = true;
     h->drawn = true;
@@ -262,11 +281,12 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     return true;
 }
 
-/* The queued frame done: its stage times from the events, its step counts from the pinned copy. */
+/* The queued frames done (both streams): the last one's stage times and step counts from its report. */
 static bool settle(HipRenderer* h) {
     if (!h->queued) return !h->lost;
     h->queued = false;
-    if (!ok(h, artfHipStreamSync(h->stream), "frame")) return false;
+    if (!ok(h, artfHipStreamSync(h->stream), "frame") || !ok(h, artfHipStreamSync(h->post), "frame")) return false;
+    h->busy[0] = h->busy[1] = false;
     ArtfHoleStats* st = &h->holeStats;
     /* each stage from its kernel's start to the next one's (the tone map: to its last workgroup's end);
      * with several rays a pixel the resolve and bloom level 0 are one pass, reported as the resolve */
@@ -326,7 +346,7 @@ static bool hipReadHdr(ArtfRenderer* r, float* red, float* green, float* blue) {
     const size_t bytes = (size_t)h->width * h->height * sizeof(float);
     float* dst[3] = {red, green, blue};
     for (int c = 0; c < 3; ++c)
-        if (!ok(h, artfHipDownload(dst[c], h->hdr[c], bytes, h->stream), "read back")) return false;
+        if (!ok(h, artfHipDownload(dst[c], h->hdr[h->set][c], bytes, h->stream), "read back")) return false;
     return true;
 }
 
@@ -361,14 +381,17 @@ static void hipSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* t) {
 static void hipDestroy(ArtfRenderer* r) {
     HipRenderer* h = (HipRenderer*)r;
     if (h->stream) artfHipStreamSync(h->stream);
-    freePlanes(h->hdr);
-    freePlanes(h->rays);
-    freePlanes(h->bloom);
+    if (h->post) artfHipStreamSync(h->post);
+    for (int k = 0; k < 2; ++k) {
+        freePlanes(h->hdr[k]), freePlanes(h->bloom[k]), freePlanes(h->rays[k]);
+        artfHipFree(h->counters[k]);
+        artfHipEventDestroy(h->handoff[k]), artfHipEventDestroy(h->done[k]);
+    }
     artfHipFree(h->bgra);
-    artfHipFree(h->counters);
     artfHipFree(h->map);
     artfHipFree(h->fluid);
     artfHipStreamDestroy(h->stream);
+    artfHipStreamDestroy(h->post);
     artfFree(h->overlay);
     artfHipHostFree(h->report);
     artfFree(h);
@@ -392,9 +415,16 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     h->base.backend = ARTF_BACKEND_HIP;
     h->device = o->device;
     h->stream = artfHipStreamCreate();
+    h->post = artfHipStreamCreate();
+    bool ready = h->stream && h->post;
+    for (int k = 0; k < 2; ++k) {
+        h->handoff[k] = artfHipEventCreateOrder();
+        h->done[k] = artfHipEventCreateOrder();
+        ready = ready && h->handoff[k] && h->done[k];
+    }
     h->clockHz = artfHipWallClockHz(o->device);
-    if (!h->stream || !(h->clockHz > 0)) {
-        artfLog("artful: HIP device %d: no stream or clock", o->device);
+    if (!ready || !(h->clockHz > 0)) {
+        artfLog("artful: HIP device %d: no streams, events or clock", o->device);
         hipDestroy(&h->base);
         return NULL;
     }

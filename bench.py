@@ -420,7 +420,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50, help="untimed frames first (50: ~60 ms, the GPU's clock up to speed)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="only the FP32 frames (for rocprofv3 runs)")
+    ap.add_argument("--profile", action="store_true", help="only the FP32 frames and the FP32 kernel timing (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
     ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge"], help="run one extra leg alone and print its JSON")
@@ -462,7 +462,9 @@ def main():
     value = rays * args.steps / fp32["wall_s"]
     log(f"fp32: {fp32}")
     if args.profile:
-        print(json.dumps({"rays_per_s": value, "trace_ms": fp32["trace_ms"], "steps": fp32["steps_per_frame"]}))
+        ms = kernel_ms(native, scene, "fp32")
+        print(json.dumps({"rays_per_s": value, "ms_per_step": fp32["wall_s"] / args.steps * 1e3, "trace_kernel_ms": ms,
+                          "achieved_tflops": fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0] / (ms * 1e-3) / 1e12}))
         return
     hdr32 = renderer.read_hdr()
 

@@ -42,6 +42,13 @@ int artfHipStreamSync(void* stream);
 void* artfHipEventCreate(void);
 void artfHipEventDestroy(void* event);
 int artfHipEventRecord(void* event, void* stream);
+/* An event that orders one stream after another and nothing more: no timing, a device-scope release. */
+void* artfHipEventCreateOrder(void);
+/* The stream's later work waits for the event's last recording. */
+int artfHipStreamWait(void* stream, void* event);
+/* 1 when the event's last recording has completed (or it was never recorded), 0 while it has not, -1 on
+ * an error. */
+int artfHipEventDone(void* event);
 /* Milliseconds between two recorded events (both must have completed). */
 float artfHipEventMs(void* start, void* stop);
 /* The last error the runtime saw, as text. */
@@ -242,13 +249,23 @@ typedef struct ArtfHipFrameArgs {
     float* hdr[3];   /* device, width * height each */
     float* bloom[3]; /* device, the pyramid as artfHipBloom lays it out */
     uint32_t* bgra;  /* device, width * height words */
-    /* NULL, or host memory from artfHipHostMalloc, ARTF_HIP_REPORT_WORDS words, there once the stream is
+    /* NULL, or host memory from artfHipHostMalloc, ARTF_HIP_REPORT_WORDS words, there once the streams are
      * synchronised: [0] the frame's steps and [1] lane-slots (the sums of the counters' halves); the GPU's
      * clock when [2] the trace started, [3] the resolve with level 0 started (with one ray a pixel: level
      * 0), [4] the rest of the bloom started (with no more levels: the tone map), [5] the tone map started,
      * [6] the tone map's last workgroup ended. Frames queued one after another share it: it holds the
      * last one's. */
     unsigned long long* report;
+    /* NULL: the whole frame on the call's stream. Else the trace alone goes on the call's stream, and the
+     * resolve, the bloom and the tone map on this second one, after `handoff` (artfHipEventCreateOrder)
+     * is recorded past the trace; `done` (the same kind, or NULL) is recorded after them. The call's
+     * stream is then free for the next frame's trace while they run: the caller keeps later frames off
+     * what they still read and write (the rays' planes, the counters, the HDR and the pyramid: a second
+     * set for the next frame, its stream waiting for `done` before a set is used again), and waits for
+     * both streams before reading the frame back. */
+    void* postStream;
+    void* handoff;
+    void* done;
 } ArtfHipFrameArgs;
 
 int artfHipDrawFrame(const ArtfHipFrameArgs* args, void* stream);

@@ -27,7 +27,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.BlackHoleDesc) == 92
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
-    assert C.sizeof(abi.HipTraceArgs) == 808 and C.sizeof(abi.HipFrameArgs) == 872
+    assert C.sizeof(abi.HipTraceArgs) == 808 and C.sizeof(abi.HipFrameArgs) == 896
     assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 96 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
     assert abi.MAP_FLOATS == 2887424
 
@@ -85,6 +85,9 @@ def test_frame_rejects_bad_arguments(native):
     a.report = 64  # a report without counters
     assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
     a.report = None
+    a.postStream = 64  # a second stream without the handoff event
+    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
+    a.postStream = None
     a.trace.rowBegin = 4  # a band, not a frame
     assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
     a.trace.rowBegin = 0

@@ -119,26 +119,31 @@ def frame_args(L, dev, s, w, h, f, bloom_pixels):
     return a
 
 
+@pytest.mark.parametrize("streams", [1, 2])
 @pytest.mark.parametrize("name,w,h", FRAMES)
-def test_frame_equals_oracle(name, w, h, gpu, dev, scene_of, oracle):
-    """Two frames in a row through artfHipDrawFrame: HDR, pyramid and BGRA equal the oracle's render, the
-    reported counts equal what artfHipTrace counts for the frame, the counters are left zero, and the
-    stage stamps are in order."""
+def test_frame_equals_oracle(name, w, h, streams, gpu, dev, scene_of, oracle):
+    """Two frames in a row through artfHipDrawFrame, on one stream or with the post passes handed to a
+    second: HDR, pyramid and BGRA equal the oracle's render, the reported counts equal what artfHipTrace
+    counts for the frame, the counters are left zero, and the stage stamps are in order."""
     L = gpu.lib()
     s = scene_of(name)
     f, bloom_pixels = s.frame(w, h)
-    print(f"frame {name} {w}x{h} side {f.size[2]} levels {f.size[3]}", flush=True)
+    print(f"frame {name} {w}x{h} side {f.size[2]} levels {f.size[3]} streams {streams}", flush=True)
     a = frame_args(L, dev, s, w, h, f, bloom_pixels)
     report = L.artfHipHostMalloc(8 * abi.REPORT_WORDS)
+    post, handoff, done = (L.artfHipStreamCreate(), L.artfHipEventCreateOrder(), L.artfHipEventCreateOrder()) if streams == 2 else (None,) * 3
     try:
         words = (C.c_uint64 * abi.REPORT_WORDS).from_address(report)
         a.report = report
+        a.postStream, a.handoff, a.done = post, handoff, done
         counts = []
         for _ in range(2):
             for i in range(abi.REPORT_WORDS):
                 words[i] = 0
             assert L.artfHipDrawFrame(C.byref(a), None) == 0
             assert L.artfHipStreamSync(None) == 0
+            if post:
+                assert L.artfHipStreamSync(post) == 0 and L.artfHipEventDone(done) == 1
             counts.append((words[0], words[1]))
             stamps = list(words[2:])
             print("clock stamps", [x - stamps[0] for x in stamps])
@@ -163,3 +168,45 @@ def test_frame_equals_oracle(name, w, h, gpu, dev, scene_of, oracle):
         assert 0 < counts[0][0] <= counts[0][1]
     finally:
         L.artfHipHostFree(report)
+        if post:
+            L.artfHipEventDestroy(handoff), L.artfHipEventDestroy(done), L.artfHipStreamDestroy(post)
+
+
+@pytest.mark.parametrize("name,w,h", [("moving_eye", 101, 67), ("config1", 320, 180), ("inner_given", 16, 9)])
+def test_renderer_frames_back_to_back(name, w, h, gpu, scene_of):
+    """The renderer queues frames without waiting, each frame's bloom and tone map running beside the next
+    frame's trace on a second stream, frames taking turns with two sets of HDR and pyramid. Frames of a
+    changing scene drawn back to back, read now and then, equal the same frames drawn one at a time by a
+    fresh renderer, bit for bit."""
+    from tests.golden import cases
+    desc, cam = cases.make(cases.CASES[name])
+    times = [desc.time + .7 * k for k in range(7)]
+
+    def scene_at(t):
+        d = abi.BlackHoleDesc.from_buffer_copy(desc)
+        d.time = t
+        return d
+
+    scenes = [gpu.Scene(scene_at(t), cam) for t in times]
+    alone = {}
+    r1 = gpu.Renderer(w, h)
+    for k in (1, 4, 6):
+        assert r1.draw(scenes[k])
+        st = r1.hole_stats()
+        alone[k] = (r1.read_pixels(), r1.read_hdr(), (st.steps, st.laneSlots))
+    r1.close()
+
+    r2 = gpu.Renderer(w, h)
+    for k, sc in enumerate(scenes):
+        assert r2.draw(sc)
+        if k in alone:
+            print(f"frame {k} at time {times[k]}", flush=True)
+            px, hdr, counts = alone[k]
+            np.testing.assert_array_equal(r2.read_hdr().view(np.uint32), hdr.view(np.uint32))
+            np.testing.assert_array_equal(r2.read_pixels(), px)
+            st = r2.hole_stats()
+            assert (st.steps, st.laneSlots) == counts
+            assert st.traceMs > 0 and st.bloomMs >= 0 and st.toneMs > 0
+    r2.close()
+    for sc in scenes:
+        sc.close()
