@@ -5,10 +5,12 @@
  * map, uploaded when mapVersion changes; the fluid field, when fluidVersion changes; the frame's
  * uniforms from artfHoleFrame), and per frame runs trace -> resolve with bloom level 0 -> the rest of
  * the bloom -> tone map (artfHipDrawFrame), its stages timed by the GPU's clock, which the kernels
- * stamp as they start. The trace runs on one stream and the resolve, bloom and tone map on a second,
- * so a frame's post passes run beside the next frame's trace; frames take turns with two sets of rays'
- * planes, counters, HDR and pyramid, and a set is used again only after the frame before last is done
- * with it (the trace's stream waits for its tone map's event when it has not ended yet).
+ * stamp as they start. Frames take turns with two sets of rays' planes, counters, HDR and pyramid, and
+ * each set has its own stream for its traces; the resolves, blooms and tone maps go on a third. So a
+ * frame's trace starts as the last one's drains, and a frame's post passes run beside the next frame's
+ * trace. A set is used again only after the frame before last is done with it (its stream waits for
+ * that frame's tone map when it has not ended yet), and the map and fluid are replaced only after the
+ * last frame's trace (the stream that uploads waits for it).
  * It is plain C over include/artful_hip.h.
  *
  * A draw only queues its frame (its step counts summed into pinned memory by its last kernel); the
@@ -32,8 +34,8 @@ int artfHipCallsLeft = -1;
 typedef struct HipRenderer {
     ArtfRenderer base;
     int device;
-    void* stream;   /* the traces */
-    void* post;     /* the resolves, blooms and tone maps */
+    void* traces[2]; /* per set: its frames' traces (and uploads) */
+    void* post;      /* the resolves, blooms and tone maps */
     void* handoff[2], *done[2]; /* per set: its trace ended; its post passes ended */
     bool busy[2];   /* per set: a frame's post passes may still use it */
     int set;        /* the set of the last frame */
@@ -164,14 +166,19 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
         h->counters[k] = artfHipMalloc(bytes);
         if (!h->counters[k]) return ok(h, 2, "allocate");
         /* zero once: each frame's resolve leaves them zero for the set's next frame */
-        if (!ok(h, artfHipMemset(h->counters[k], 0, bytes, h->stream), "clear") || !ok(h, artfHipStreamSync(h->stream), "clear")) return false;
+        if (!ok(h, artfHipMemset(h->counters[k], 0, bytes, h->traces[k]), "clear") || !ok(h, artfHipStreamSync(h->traces[k]), "clear")) return false;
     }
     return true;
 }
 
-/* The map and the fluid on the device, as the scene now has them. */
-static bool syncScene(HipRenderer* h, const ArtfScene* s) {
+/* Before the map or fluid the traces read is replaced on set's stream: that stream waits for the other
+ * set's last trace (its own are in order already). */
+static bool uploadAfterTraces(HipRenderer* h, int set) { return ok(h, artfHipStreamWait(h->traces[set], h->handoff[set ^ 1]), "upload order"); }
+
+/* The map and the fluid on the device, as the scene now has them, uploaded on set's stream. */
+static bool syncScene(HipRenderer* h, const ArtfScene* s, int set) {
     const ArtfHole* hole = &s->hole;
+    void* stream = h->traces[set];
     if (!h->map) {
         h->map = artfHipMalloc((size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float));
         h->fluid = artfHipMalloc((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float));
@@ -181,15 +188,16 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s) {
         h->mapScene = 0; /* the scene's map is on this device: read in place */
     } else if (h->mapScene != s->id || h->mapVersion != hole->mapVersion) {
         const size_t bytes = (size_t)ARTF_HOLE_MAP_FLOATS * sizeof(float);
+        if (!uploadAfterTraces(h, set)) return false;
         int status;
         if (hole->deviceMap) { /* on another device: through the host */
             float* staged = artfMalloc(bytes);
             status = artfHipDownload(staged, hole->deviceMap, bytes, NULL);
-            if (!status) status = artfHipUpload(h->map, staged, bytes, h->stream);
-            if (!status) status = artfHipStreamSync(h->stream);
+            if (!status) status = artfHipUpload(h->map, staged, bytes, stream);
+            if (!status) status = artfHipStreamSync(stream);
             artfFree(staged);
         } else {
-            status = artfHipUpload(h->map, hole->map, bytes, h->stream);
+            status = artfHipUpload(h->map, hole->map, bytes, stream);
         }
         if (!ok(h, status, "map upload")) return false;
         h->mapScene = s->id;
@@ -203,14 +211,15 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s) {
     }
     if (h->fluidScene != s->id || h->fluidVersion != hole->fluidVersion) {
         const size_t bytes = (size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT * sizeof(float);
+        if (!uploadAfterTraces(h, set)) return false;
         int status;
         if (hole->deviceFluid) { /* on another device: through the host */
             float* field = artfMalloc(bytes);
-            status = artfDeviceFluidRead(hole->deviceFluid, field) ? artfHipUpload(h->fluid, field, bytes, h->stream) : 1;
-            if (!status) status = artfHipStreamSync(h->stream);
+            status = artfDeviceFluidRead(hole->deviceFluid, field) ? artfHipUpload(h->fluid, field, bytes, stream) : 1;
+            if (!status) status = artfHipStreamSync(stream);
             artfFree(field);
         } else {
-            status = artfHipUpload(h->fluid, artfFluidField(hole->fluid), bytes, h->stream);
+            status = artfHipUpload(h->fluid, artfFluidField(hole->fluid), bytes, stream);
         }
         if (!ok(h, status, "fluid upload")) return false;
         h->fluidScene = s->id;
@@ -238,7 +247,9 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     const int side = (int)h->frame.size[2];
     const size_t rays = (size_t)h->width * h->height * side * side;
     if (!ensure(h, bloomPixels, rays)) return false;
-    if (!syncScene(h, scene)) return false;
+    /* the other set than the last frame's, once the frame before last is done with it */
+    const int set = h->set ^ 1;
+    if (!syncScene(h, scene, set)) return false;
 
     ArtfHipFrameArgs fa;
     memset(&fa, 0, sizeof fa);
@@ -247,12 +258,10 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a->frame64);
     a->map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
     a->fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
-    /* the other set than the last frame's, once the frame before last is done with it */
-    const int set = h->set ^ 1;
     if (h->busy[set]) {
         const int done = artfHipEventDone(h->done[set]);
         if (done < 0) return ok(h, 1, "frame order");
-        if (!done && !ok(h, artfHipStreamWait(h->stream, h->done[set]), "frame order")) return false;
+        if (!done && !ok(h, artfHipStreamWait(h->traces[set], h->done[set]), "frame order")) return false;
     }
     float** out = side == 1 ? h->hdr[set] : h->rays[set];
     for (int c = 0; c < 3; ++c) a->out[c] = out[c], fa.hdr[c] = h->hdr[set][c], fa.bloom[c] = h->bloom[set][c];
@@ -268,7 +277,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     fa.postStream = h->post;
     fa.handoff = h->handoff[set];
     fa.done = h->done[set];
-    if (!ok(h, artfHipDrawFrame(&fa, h->stream), "frame")) return false;
+    if (!ok(h, artfHipDrawFrame(&fa, h->traces[set]), "frame")) return false;
     h->busy[set] = true;
     h->set = set;
     h->frameRays = rays;
@@ -285,7 +294,9 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
 static bool settle(HipRenderer* h) {
     if (!h->queued) return !h->lost;
     h->queued = false;
-    if (!ok(h, artfHipStreamSync(h->stream), "frame") || !ok(h, artfHipStreamSync(h->post), "frame")) return false;
+    for (int k = 0; k < 2; ++k)
+        if (!ok(h, artfHipStreamSync(h->traces[k]), "frame")) return false;
+    if (!ok(h, artfHipStreamSync(h->post), "frame")) return false;
     h->busy[0] = h->busy[1] = false;
     ArtfHoleStats* st = &h->holeStats;
     /* each stage from its kernel's start to the next one's (the tone map: to its last workgroup's end);
@@ -333,7 +344,7 @@ static bool hipReadPixels(ArtfRenderer* r, uint32_t* dst) {
     const size_t pixels = (size_t)h->width * h->height;
     if (!h->hole) {
         for (size_t i = 0; i < pixels; ++i) dst[i] = h->background;
-    } else if (!ok(h, artfHipDownload(dst, h->bgra, pixels * 4, h->stream), "read back")) {
+    } else if (!ok(h, artfHipDownload(dst, h->bgra, pixels * 4, h->post), "read back")) {
         return false;
     }
     compose(h, dst);
@@ -346,7 +357,7 @@ static bool hipReadHdr(ArtfRenderer* r, float* red, float* green, float* blue) {
     const size_t bytes = (size_t)h->width * h->height * sizeof(float);
     float* dst[3] = {red, green, blue};
     for (int c = 0; c < 3; ++c)
-        if (!ok(h, artfHipDownload(dst[c], h->hdr[h->set][c], bytes, h->stream), "read back")) return false;
+        if (!ok(h, artfHipDownload(dst[c], h->hdr[h->set][c], bytes, h->post), "read back")) return false;
     return true;
 }
 
@@ -380,7 +391,8 @@ static void hipSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* t) {
 
 static void hipDestroy(ArtfRenderer* r) {
     HipRenderer* h = (HipRenderer*)r;
-    if (h->stream) artfHipStreamSync(h->stream);
+    for (int k = 0; k < 2; ++k)
+        if (h->traces[k]) artfHipStreamSync(h->traces[k]);
     if (h->post) artfHipStreamSync(h->post);
     for (int k = 0; k < 2; ++k) {
         freePlanes(h->hdr[k]), freePlanes(h->bloom[k]), freePlanes(h->rays[k]);
@@ -390,7 +402,7 @@ static void hipDestroy(ArtfRenderer* r) {
     artfHipFree(h->bgra);
     artfHipFree(h->map);
     artfHipFree(h->fluid);
-    artfHipStreamDestroy(h->stream);
+    artfHipStreamDestroy(h->traces[0]), artfHipStreamDestroy(h->traces[1]);
     artfHipStreamDestroy(h->post);
     artfFree(h->overlay);
     artfHipHostFree(h->report);
@@ -414,9 +426,10 @@ ArtfRenderer* artfHipNew(const ArtfRendererOptions* o) {
     h->base.vt = &hipTable;
     h->base.backend = ARTF_BACKEND_HIP;
     h->device = o->device;
-    h->stream = artfHipStreamCreate();
+    h->traces[0] = artfHipStreamCreate();
+    h->traces[1] = artfHipStreamCreate();
     h->post = artfHipStreamCreate();
-    bool ready = h->stream && h->post;
+    bool ready = h->traces[0] && h->traces[1] && h->post;
     for (int k = 0; k < 2; ++k) {
         h->handoff[k] = artfHipEventCreateOrder();
         h->done[k] = artfHipEventCreateOrder();
