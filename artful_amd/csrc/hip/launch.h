@@ -12,10 +12,11 @@ namespace artf {
 constexpr int kTile = 16;
 constexpr int kGroup = 256;
 
-/* Waves per SIMD the tracers are built for: 4 for the FP32 ones (at most 128 VGPRs, which they fit
- * in); the FP64 and mixed ones would take 250 VGPRs for 2 waves, and are held to 168 for 3, which
- * spills a little in the rarely taken FP64 paths and measures 5% faster (4 spills too much). */
-constexpr int wavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP64 || precision == ARTF_PRECISION_MIXED ? 3 : 4; }
+/* Waves per SIMD the tracers are built for: 4 for the FP32 and mixed ones (at most 128 VGPRs, which
+ * they fit in; the mixed tile kernel measured 10% faster at 4 than at 3); the FP64 one would take 250
+ * VGPRs for 2 waves, and is held to 168 for 3, which spills a little in its rarely taken paths and
+ * measures 5% faster (4 spills too much). */
+constexpr int wavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP64 ? 3 : 4; }
 
 /* The GPU's clock (wall_clock64: artfHipWallClockHz ticks a second) stored to *at by the launch's first
  * lane of its first workgroup as it starts, with a system-scope store (at may be host memory): a stage

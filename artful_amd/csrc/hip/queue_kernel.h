@@ -140,8 +140,8 @@ struct MixedLane {
     RayConst<double> cd;
     RayConst<float> cf;
     RayState<float> s;
-    Gathered<double> acc;
-    Away<double> away;
+    Gathered<float> acc;
+    Away<float> away;
 
     __device__ __forceinline__ void begin(const WaveView& w, double nx, double ny) {
         Eye<double> e;
@@ -150,49 +150,49 @@ struct MixedLane {
         setupRay(e, nx, ny, cd, sd);
         cf = narrowConst(cd);
         s = castState<float>(sd);
-        acc.r = acc.g = acc.b = 0.0, acc.through = 1.0;
+        acc.r = acc.g = acc.b = 0.f, acc.through = 1.f;
     }
 
+    /* traceRayMixed's step: FP64 again at turning points and near the horizon, the rest in FP32 */
     __device__ __forceinline__ bool step(const WaveView& w, float time, bool& sky) {
         const ArtfHipView& v = *w.v;
-        const double uHorizon = v.frame64.spin[1], uEscape = (double)w.uEscape;
-        const float uNear = (float)(.9 * uHorizon), uEscapeF = w.uEscape;
+        const float uHorizon = (float)v.frame64.spin[1], uEscape = w.uEscape;
+        const float uNear = (float)(.9 * v.frame64.spin[1]);
         RayState<float> n;
         StepLooks<float> looks;
         nystromStep(cf, s, n, looks);
-        const bool special = s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || s.n2 * n.n2 <= 0.f || n.u > uNear || s.u > uNear || !(n.u > uEscapeF);
-        if (!special) {
-            s = n;
-            return false;
+        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uNear || s.u > uNear) {
+            const RayState<double> x = castState<double>(s);
+            RayState<double> m;
+            StepLooks<double> l;
+            nystromStep(cd, x, m, l);
+            n = castState<float>(m);
+            looks.k1 = (float)l.k1, looks.a1x = (float)l.a1x, looks.a1y = (float)l.a1y, looks.a1z = (float)l.a1z;
         }
-        const RayState<double> x = castState<double>(s);
-        RayState<double> m;
-        StepLooks<double> l;
-        nystromStep(cd, x, m, l);
-        if (x.n2 * m.n2 < 0.0) {
-            DiscConst<double> d;
+        if (s.n2 * n.n2 < 0.f) {
+            DiscConst<float> d;
             discOfView(v, time, d);
-            if (v.disc == DISC_PAINTED) crossPainted(d, cd, x, m, l, w.tables, acc);
-            else crossAnalytic(d, cd, x, m, l, v.emissivity, w.tables.map, acc);
+            if (v.disc == DISC_PAINTED) crossPainted(d, cf, s, n, looks, w.tables, acc);
+            else crossAnalytic(d, cf, s, n, looks, v.emissivity, w.tables.map, acc);
         }
-        const bool escaped = uEscape > m.u;
+        const bool escaped = uEscape > n.u;
         if (escaped) {
-            const double part = quot(x.u - uEscape, x.u - m.u);
-            away.x = x.n0 + part * (m.n0 - x.n0);
-            away.y = x.n1 + part * (m.n1 - x.n1);
-            away.z = x.n2 + part * (m.n2 - x.n2);
-            away.psi = x.psi + part * (m.psi - x.psi);
+            const float part = quot(s.u - uEscape, s.u - n.u);
+            away.x = s.n0 + part * (n.n0 - s.n0);
+            away.y = s.n1 + part * (n.n1 - s.n1);
+            away.z = s.n2 + part * (n.n2 - s.n2);
+            away.psi = s.psi + part * (n.psi - s.psi);
             sky = true;
         }
-        if (escaped || m.u > uHorizon || !(acc.through > .004)) return true;
-        s = castState<float>(m);
+        if (escaped || n.u > uHorizon || !(acc.through > .004f)) return true;
+        s = n;
         return false;
     }
 
     __device__ __forceinline__ void finish(const WaveView& w, bool sky, float rgb[3]) {
         const float stars = w.v->frame.look[2];
-        if (stars > 0.f && sky) skyLight(stars, cd, away.x, away.y, away.z, away.psi, (double)w.v->frame.look[1], w.tables.map, acc);
-        rgb[0] = (float)acc.r, rgb[1] = (float)acc.g, rgb[2] = (float)acc.b;
+        if (stars > 0.f && sky) skyLight(stars, cf, away.x, away.y, away.z, away.psi, w.v->frame.look[1], w.tables.map, acc);
+        rgb[0] = acc.r, rgb[1] = acc.g, rgb[2] = acc.b;
     }
 };
 

@@ -76,6 +76,23 @@ def test_precisions_against_oracle64(renderer, scene_of, oracle, precision, mean
     assert rel.mean() < mean and (rel > 1e-3).mean() < share
 
 
+def test_mixed_closer_to_fp64_than_fp32(renderer, scene_of, oracle):
+    """The mixed tracer (FP64 at the setup, the turning points and near the horizon) exists to be nearer
+    the FP64 result than the FP32 tracer: its mean relative error and its share of pixels past 1e-3
+    against the FP64 oracle are the smaller, on two views."""
+    for name, (w, h) in (("config1", (320, 180)), ("config2_small", (192, 108))):
+        s = scene_of(name)
+        f, _ = s.frame(w, h)
+        want = oracle.trace(f, s.map(), s.fluid()[0], 64, s.frame64(w, h))
+        err = {}
+        for precision in ("fp32", "mixed"):
+            hdr, _ = draw(renderer, s, w, h, precision=precision)
+            rel = relative(hdr, want)
+            err[precision] = (rel.mean(), (rel > 1e-3).mean())
+        print(name, err)
+        assert err["mixed"][0] < err["fp32"][0] and err["mixed"][1] <= err["fp32"][1]
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp32_fast", "mixed", "fp64"])
 def test_analytic_disc(renderer, native, oracle, precision):
     """The analytic r^-q disc (BASELINE config 1's extension: 2000 steps, escape at r > 100) against the
