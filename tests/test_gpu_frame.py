@@ -210,3 +210,30 @@ def test_renderer_frames_back_to_back(name, w, h, gpu, scene_of):
     r2.close()
     for sc in scenes:
         sc.close()
+
+
+def test_renderer_resizes_and_precisions_between_queued_frames(gpu, scene_of):
+    """Frames queued back to back across resizes (buffers replaced while frames may be in flight) and
+    precision changes: each frame read back equals a fresh renderer's frame of the same size and
+    tracing, bit for bit."""
+    s = scene_of("moving_eye")
+    plan = [((101, 67), "fp32"), ((320, 180), "fp32"), ((64, 36), "mixed"), ((320, 180), "fp32"), ((3, 2), "fp32_fast"), ((101, 67), "fp64")]
+    want = []
+    for (w, h), p in plan:
+        r = gpu.Renderer(w, h)
+        r.set_tracing(p)
+        assert r.draw(s)
+        want.append((r.read_pixels(), r.read_hdr()))
+        r.close()
+    r = gpu.Renderer(8, 8)
+    for k, ((w, h), p) in enumerate(plan):
+        r.resize(w, h)
+        r.set_tracing(p)
+        for _ in range(3):  # queued, not waited for
+            assert r.draw(s)
+        print(f"step {k}: {w}x{h} {p}", flush=True)
+        np.testing.assert_array_equal(r.read_pixels(), want[k][0])
+        np.testing.assert_array_equal(r.read_hdr().view(np.uint32), want[k][1].view(np.uint32))
+        for _ in range(2):  # still in flight when the next step resizes
+            assert r.draw(s)
+    r.close()
