@@ -237,3 +237,27 @@ def test_renderer_resizes_and_precisions_between_queued_frames(gpu, scene_of):
         for _ in range(2):  # still in flight when the next step resizes
             assert r.draw(s)
     r.close()
+
+
+def test_renderer_hole_removed_and_restored_between_queued_frames(gpu, native, scene_of):
+    """Queued frames, then the hole taken out of the scene (the frame is the background), then put back:
+    the frame after equals a fresh renderer's (renderer.h's scene contract, scene.h:497-500)."""
+    from tests.golden import cases
+    desc, cam = cases.make(cases.CASES["config1"])
+    s = native.Scene(desc, cam)
+    r0 = gpu.Renderer(64, 36)
+    assert r0.draw(s)
+    want = r0.read_pixels()
+    r0.close()
+    r = gpu.Renderer(64, 36)
+    for _ in range(3):
+        assert r.draw(s)
+    s.set_hole(None)
+    assert r.draw(s)
+    assert (r.read_pixels() == 0xFF0D0F14).all()
+    s.set_hole(desc)
+    for _ in range(2):
+        assert r.draw(s)
+    np.testing.assert_array_equal(r.read_pixels(), want)
+    r.close()
+    s.close()
