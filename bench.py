@@ -457,6 +457,10 @@ def main():
     renderer = native.Renderer(WIDTH, HEIGHT)
     log(renderer.description())
 
+    # the dominant kernel alone, timed with HIP events on its stream: the roofline's and the tracers' rates
+    # (first, so the frames below start with the GPU's clock up from the idle of setting up)
+    kernel = None if args.profile else {p: kernel_ms(native, scene, p) for p in ("fp32", "fp32_fast", "mixed", "fp64")}
+    log(f"kernel ms: {kernel}")
     fp32 = time_frames(native, renderer, scene, "fp32", args.steps, args.warmup)
     rays = fp32["rays"]
     value = rays * args.steps / fp32["wall_s"]
@@ -477,9 +481,6 @@ def main():
     ref64 = hdrs["fp64"].astype(np.float64)
     errors = {p: rel_error(h, ref64) for p, h in (("fp32", hdr32), ("fp32_fast", hdrs["fp32_fast"]), ("mixed", hdrs["mixed"]))}
 
-    # the dominant kernel alone, timed with HIP events on its stream: the roofline's and the tracers' rates
-    kernel = {p: kernel_ms(native, scene, p) for p in ("fp32", "fp32_fast", "mixed", "fp64")}
-    log(f"kernel ms: {kernel}")
     flops = fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0]
     achieved = flops / (kernel["fp32"] * 1e-3) / 1e12
     traffic = traffic_from_profiles()
