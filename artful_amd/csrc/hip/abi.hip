@@ -81,6 +81,8 @@ void artfHipStreamDestroy(void* stream) {
 
 int artfHipStreamSync(void* stream) { return (int)hipStreamSynchronize((hipStream_t)stream); }
 
+int artfHipDeviceSync(void) { return (int)hipDeviceSynchronize(); }
+
 void* artfHipEventCreate(void) {
     hipEvent_t e = NULL;
     if (hipEventCreate(&e) != hipSuccess) return NULL;

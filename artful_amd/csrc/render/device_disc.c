@@ -37,11 +37,18 @@ float* artfDeviceMapNew(int device) {
 bool artfDeviceMapRings(float* map, float inner, float outer, float spin, int device) {
     float rows[ARTF_HOLE_MAP_HEIGHT * 2], glow[ARTF_HOLE_MAP_HEIGHT];
     artfDiscRingTables(rows, glow, inner, outer, spin);
-    return !artfHipSetDevice(device) && !artfHipUpload(map + ARTF_HOLE_ROWS_AT, rows, sizeof rows, NULL) &&
+    /* a renderer may still be tracing through the old rings: the device's queued work first */
+    return !artfHipSetDevice(device) && !artfHipDeviceSync() && !artfHipUpload(map + ARTF_HOLE_ROWS_AT, rows, sizeof rows, NULL) &&
            !artfHipUpload(map + ARTF_HOLE_GLOW_AT, glow, sizeof glow, NULL) && !artfHipStreamSync(NULL);
 }
 
-void artfDeviceMapFree(float* map) { artfHipFree(map); }
+void artfDeviceMapFree(float* map) {
+    if (!map) return;
+    artfHipDeviceSync();
+    artfHipFree(map);
+}
+
+#define READERS 8 /* marked reads of a field buffer kept before the oldest is waited for at once */
 
 struct ArtfDeviceFluid {
     int device;
@@ -50,19 +57,31 @@ struct ArtfDeviceFluid {
     ArtfFluidRings rings;
     uint32_t rng;
     double time;
-    void* block; /* device: the seven fields, then the ring constants */
+    void* block; /* device: six fields, then two field buffers, then the ring constants */
     ArtfHipFluid f;
     ArtfFluidEddy* eddies; /* device, room for eddyRoom */
     int eddyRoom;
     ArtfFluidEddy* eddiesHost;
+    /* the field, in two buffers taken in turn by queued advances: the current one, the event after
+     * each buffer's last write, and the marked reads of each since */
+    float* fields[2];
+    int current;
+    void* written[2];
+    void* reads[2][READERS];
+    int readCount[2];
 };
 
 void artfDeviceFluidFree(ArtfDeviceFluid* f) {
     if (!f) return;
-    if (f->stream) artfHipStreamSync(f->stream);
+    /* a renderer may still be tracing through the field */
+    if (!artfHipSetDevice(f->device)) artfHipDeviceSync();
     artfHipFree(f->block);
     artfHipFree(f->eddies);
     artfFree(f->eddiesHost);
+    for (int b = 0; b < 2; ++b) {
+        artfHipEventDestroy(f->written[b]);
+        for (int k = 0; k < READERS; ++k) artfHipEventDestroy(f->reads[b][k]);
+    }
     artfHipStreamDestroy(f->stream);
     artfFree(f);
 }
@@ -94,7 +113,18 @@ static bool run(ArtfDeviceFluid* f, int steps) {
     return true;
 }
 
-static bool finish(ArtfDeviceFluid* f) { return !artfHipFluidField(&f->f, f->stream) && !artfHipStreamSync(f->stream); }
+/* The field of the fluid as stepped, into the other buffer once its marked reads have ended; waited
+ * for here when `wait`. */
+static bool finish(ArtfDeviceFluid* f, bool wait) {
+    const int b = f->current ^ 1;
+    for (int k = 0; k < f->readCount[b]; ++k)
+        if (artfHipStreamWait(f->stream, f->reads[b][k])) return false;
+    f->readCount[b] = 0;
+    f->f.field = f->fields[b];
+    if (artfHipFluidField(&f->f, f->stream) || artfHipEventRecord(f->written[b], f->stream)) return false;
+    f->current = b;
+    return !wait || !artfHipStreamSync(f->stream);
+}
 
 ArtfDeviceFluid* artfDeviceFluidNew(float inner, float outer, float spin, double time, int device) {
     if (device < 0 || device >= artfHipDeviceCount() || artfHipSetDevice(device)) {
@@ -105,14 +135,20 @@ ArtfDeviceFluid* artfDeviceFluidNew(float inner, float outer, float spin, double
     f->device = device, f->inner = inner, f->outer = outer, f->rng = ARTF_FLUID_SEED, f->time = time;
     artfFluidRingsOf(inner, outer, spin, &f->rings);
     f->stream = artfHipStreamCreate();
-    f->block = artfHipMalloc(7 * CELLS * sizeof(float) + sizeof(ArtfFluidRings));
+    f->block = artfHipMalloc(8 * CELLS * sizeof(float) + sizeof(ArtfFluidRings));
     bool ready = f->stream && f->block;
+    for (int b = 0; b < 2; ++b) {
+        ready = ready && (f->written[b] = artfHipEventCreateOrder()) != NULL;
+        for (int k = 0; k < READERS; ++k) ready = ready && (f->reads[b][k] = artfHipEventCreateOrder()) != NULL;
+    }
     if (ready) {
         float* p = (float*)f->block;
         f->f.density = p, f->f.radial = p + CELLS, f->f.around = p + 2 * CELLS;
         for (int k = 0; k < 3; ++k) f->f.scratch[k] = p + (3 + k) * CELLS;
-        f->f.field = p + 6 * CELLS;
-        f->f.rings = (const ArtfFluidRings*)(p + 7 * CELLS);
+        f->fields[0] = p + 6 * CELLS, f->fields[1] = p + 7 * CELLS;
+        f->current = 1; /* the first field goes to buffer 0 */
+        f->f.field = f->fields[0];
+        f->f.rings = (const ArtfFluidRings*)(p + 8 * CELLS);
         ready = !artfHipUpload((void*)f->f.rings, &f->rings, sizeof f->rings, f->stream) && !artfHipFluidSeed(&f->f, f->stream) &&
                 eddyRoom(f, FIRST_STIRS);
     }
@@ -122,7 +158,7 @@ ArtfDeviceFluid* artfDeviceFluidNew(float inner, float outer, float spin, double
         ready = !artfHipUpload(f->eddies, f->eddiesHost, FIRST_STIRS * sizeof(ArtfFluidEddy), f->stream) &&
                 !artfHipFluidStir(&f->f, f->eddies, FIRST_STIRS, f->stream) && run(f, ARTF_FLUID_WARM);
         f->time = time;
-        ready = ready && finish(f);
+        ready = ready && finish(f, true);
     }
     if (!ready) {
         artfLog("artful: the fluid could not be set up on the device");
@@ -132,7 +168,7 @@ ArtfDeviceFluid* artfDeviceFluidNew(float inner, float outer, float spin, double
     return f;
 }
 
-bool artfDeviceFluidAdvance(ArtfDeviceFluid* f, double time) {
+static bool advance(ArtfDeviceFluid* f, double time, bool wait) {
     if (time - f->time < ARTF_FLUID_DT) {
         /* gas does not run back; far behind, the clock is simply reset (as the host fluid) */
         if (time < f->time - 8.0) f->time = time;
@@ -143,15 +179,37 @@ bool artfDeviceFluidAdvance(ArtfDeviceFluid* f, double time) {
         f->time = time - (double)ARTF_FLUID_CATCH_UP * ARTF_FLUID_DT;
         steps = ARTF_FLUID_CATCH_UP;
     }
-    if (artfHipSetDevice(f->device) || !run(f, steps) || !finish(f)) {
+    if (artfHipSetDevice(f->device) || !run(f, steps) || !finish(f, wait)) {
         artfLog("artful: the fluid's step failed on the device");
         return false;
     }
     return true;
 }
 
-const float* artfDeviceFluidField(const ArtfDeviceFluid* f) { return f->f.field; }
+bool artfDeviceFluidAdvance(ArtfDeviceFluid* f, double time) { return advance(f, time, true); }
 
-bool artfDeviceFluidRead(const ArtfDeviceFluid* f, float* field) { return !artfHipDownload(field, f->f.field, CELLS * sizeof(float), f->stream); }
+bool artfDeviceFluidAdvanceQueued(ArtfDeviceFluid* f, double time) { return advance(f, time, false); }
+
+const float* artfDeviceFluidField(const ArtfDeviceFluid* f) { return f->fields[f->current]; }
+
+bool artfDeviceFluidReadOn(ArtfDeviceFluid* f, void* stream) { return !artfHipStreamWait(stream, f->written[f->current]); }
+
+bool artfDeviceFluidReadDone(ArtfDeviceFluid* f, void* stream) {
+    const int b = f->current;
+    if (f->readCount[b] == READERS) { /* no slot left: the fluid's stream waits for the oldest read now */
+        if (artfHipStreamWait(f->stream, f->reads[b][0])) return false;
+        for (int k = 1; k < READERS; ++k) {
+            void* e = f->reads[b][k - 1];
+            f->reads[b][k - 1] = f->reads[b][k];
+            f->reads[b][k] = e;
+        }
+        --f->readCount[b];
+    }
+    return !artfHipEventRecord(f->reads[b][f->readCount[b]++], stream);
+}
+
+bool artfDeviceFluidRead(const ArtfDeviceFluid* f, float* field) {
+    return !artfHipDownload(field, f->fields[f->current], CELLS * sizeof(float), f->stream);
+}
 
 double artfDeviceFluidTime(const ArtfDeviceFluid* f) { return f->time; }

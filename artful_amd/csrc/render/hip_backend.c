@@ -257,7 +257,11 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     a->frame = h->frame;
     if (h->tracing.precision == ARTF_PRECISION_MIXED || h->tracing.precision == ARTF_PRECISION_FP64) artfHoleFrame64(scene, h->width, h->height, &a->frame64);
     a->map = scene->hole.deviceMap && scene->hole.discDevice == h->device ? scene->hole.deviceMap : h->map;
-    a->fluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? artfDeviceFluidField(scene->hole.deviceFluid) : h->fluid;
+    /* a fluid stepped on this device: its field read in place, once written (its steps are queued on its
+     * own stream), and not written again in that buffer before this trace has read it */
+    ArtfDeviceFluid* deviceFluid = scene->hole.deviceFluid && scene->hole.discDevice == h->device ? scene->hole.deviceFluid : NULL;
+    a->fluid = deviceFluid ? artfDeviceFluidField(deviceFluid) : h->fluid;
+    if (deviceFluid && !ok(h, artfDeviceFluidReadOn(deviceFluid, h->traces[set]) ? 0 : 1, "fluid order")) return false;
     if (h->busy[set]) {
         const int done = artfHipEventDone(h->done[set]);
         if (done < 0) return ok(h, 1, "frame order");
@@ -278,14 +282,12 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     fa.handoff = h->handoff[set];
     fa.done = h->done[set];
     if (!ok(h, artfHipDrawFrame(&fa, h->traces[set]), "frame")) return false;
+    if (deviceFluid && !ok(h, artfDeviceFluidReadDone(deviceFluid, h->traces[set]) ? 0 : 1, "fluid order")) return false;
     h->busy[set] = true;
     h->set = set;
     h->frameRays = rays;
     h->queued = true;
     h->drawn = true;
-    /* a scene whose disc lives on the device changes it on streams of its own when it is next set:
-     * its frame is finished before the scene is handed back */
-    if ((scene->hole.deviceMap || scene->hole.deviceFluid) && !settle(h)) return false;
     h->cpuMs = artfNowMs() - t0;
     return true;
 }

@@ -115,7 +115,7 @@ void artfSceneSetBlackHole(ArtfScene* s, const ArtfBlackHoleDesc* desc) {
     if (!sameSpin) {
         ringMap(h);
         newFluid(h, desc->time);
-    } else if (h->deviceFluid ? artfDeviceFluidAdvance(h->deviceFluid, desc->time) : artfFluidAdvance(h->fluid, desc->time)) {
+    } else if (h->deviceFluid ? artfDeviceFluidAdvanceQueued(h->deviceFluid, desc->time) : artfFluidAdvance(h->fluid, desc->time)) {
         ++h->fluidVersion;
     }
 }

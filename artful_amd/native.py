@@ -57,7 +57,7 @@ def lib() -> C.CDLL:
         "artfHipMalloc": ([C.c_size_t], P), "artfHipFree": ([P], None),
         "artfHipUpload": ([P, P, C.c_size_t, P], I), "artfHipDownload": ([P, P, C.c_size_t, P], I),
         "artfHipMemset": ([P, I, C.c_size_t, P], I), "artfHipCopy": ([P, P, C.c_size_t, P], I),
-        "artfHipStreamCreate": ([], P), "artfHipStreamDestroy": ([P], None), "artfHipStreamSync": ([P], I),
+        "artfHipStreamCreate": ([], P), "artfHipStreamDestroy": ([P], None), "artfHipStreamSync": ([P], I), "artfHipDeviceSync": ([], I),
         "artfHipEventCreate": ([], P), "artfHipEventDestroy": ([P], None), "artfHipEventRecord": ([P, P], I),
         "artfHipEventMs": ([P, P], F), "artfHipErrorText": ([I], C.c_char_p),
         "artfHipHostMalloc": ([C.c_size_t], P), "artfHipHostFree": ([P], None),

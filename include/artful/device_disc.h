@@ -16,20 +16,32 @@ extern "C" {
  * table from the host; its per-ring rows and glow zero until artfDeviceMapRings. NULL if it cannot be
  * made there (logged). Free with artfDeviceMapFree. */
 float* artfDeviceMapNew(int device);
-/* The per-ring rows (speed, heat) and glow of this disc (setRings, hole.c:186-197) into a device map. */
+/* The per-ring rows (speed, heat) and glow of this disc (setRings, hole.c:186-197) into a device map,
+ * once the device's queued work (a renderer's trace reading the map) is done. */
 bool artfDeviceMapRings(float* map, float inner, float outer, float spin, int device);
+/* Frees it once the device's queued work is done. */
 void artfDeviceMapFree(float* map);
 
 typedef struct ArtfDeviceFluid ArtfDeviceFluid;
 
 /* As artfFluidNew(inner, outer, spin, time), on HIP device `device`; NULL if it cannot run there (logged). */
 ArtfDeviceFluid* artfDeviceFluidNew(float inner, float outer, float spin, double time, int device);
+/* Frees it once the device's queued work (a renderer's trace reading the field) is done. */
 void artfDeviceFluidFree(ArtfDeviceFluid* f);
-/* As artfFluidAdvance: true if the field changed. */
+/* As artfFluidAdvance: true if the field changed. The new field is ready when it returns. */
 bool artfDeviceFluidAdvance(ArtfDeviceFluid* f, double time);
-/* The field in device memory, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats, ready when New or Advance
- * returns. */
+/* The same, queued on the fluid's stream without waiting: the new field goes to the other of two field
+ * buffers, after every read of that buffer marked with artfDeviceFluidReadDone has ended; a reader makes
+ * its stream wait for it with artfDeviceFluidReadOn. */
+bool artfDeviceFluidAdvanceQueued(ArtfDeviceFluid* f, double time);
+/* The field in device memory, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT floats: ready when New or Advance
+ * returns, or after AdvanceQueued on a stream that artfDeviceFluidReadOn has ordered. */
 const float* artfDeviceFluidField(const ArtfDeviceFluid* f);
+/* Work queued on `stream` from now on waits for the current field to be written. */
+bool artfDeviceFluidReadOn(ArtfDeviceFluid* f, void* stream);
+/* The work queued on `stream` so far reads the current field: the fluid writes that buffer again only
+ * after it. */
+bool artfDeviceFluidReadDone(ArtfDeviceFluid* f, void* stream);
 /* The field copied to the host. */
 bool artfDeviceFluidRead(const ArtfDeviceFluid* f, float* field);
 double artfDeviceFluidTime(const ArtfDeviceFluid* f);

@@ -39,6 +39,8 @@ int artfHipCopy(void* dst, const void* src, size_t bytes, void* stream);
 void* artfHipStreamCreate(void);
 void artfHipStreamDestroy(void* stream);
 int artfHipStreamSync(void* stream);
+/* Every stream of the current device done. */
+int artfHipDeviceSync(void);
 void* artfHipEventCreate(void);
 void artfHipEventDestroy(void* event);
 int artfHipEventRecord(void* event, void* stream);
