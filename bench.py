@@ -299,16 +299,17 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=No
     return out
 
 
-def animation(native, frames=30):
+def animation(native, frames=100, warmup=50):
     """The reference driver's loop (tools/blackhole.c: time advanced .2 a frame, one fluid step) on the
     1080p view: artfSceneSetBlackHole then artfRendererDraw, per frame, wall clock, with the disc's map
     and fluid on the host (the reference's way: the fluid stepped by the CPU, then uploaded) and on the
     device (artfSceneSetDiscOnDevice); also the scene's first set-up (map paint + fluid warm-up) each
-    way, and whether the two give the same frames."""
+    way, and whether the two give the same frames. Each way first draws `warmup` frames untimed (the GPU's
+    clock comes up from idle over the first ~30), then times `frames`."""
     desc, cam = workload_scene(native)
     r = native.Renderer(WIDTH, HEIGHT)
     r.set_tracing("fp32")
-    out = {"frames": frames}
+    out = {"frames": frames, "warmup": warmup}
     last = {}
     for where in ("host", "device"):
         t0 = time.perf_counter()
@@ -319,9 +320,12 @@ def animation(native, frames=30):
         s.set_hole(desc)
         setup = time.perf_counter() - t0
         r.draw(s)
-        t0 = time.perf_counter()
         hole_ms = 0.0
-        for k in range(1, frames + 1):
+        for k in range(1, warmup + frames + 1):
+            if k == warmup + 1:
+                r.hole_stats()  # waits for the last warm-up frame
+                t0 = time.perf_counter()
+                hole_ms = 0.0
             desc.time = .2 * k
             t1 = time.perf_counter()
             s.set_hole(desc)
