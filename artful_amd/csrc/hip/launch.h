@@ -8,9 +8,13 @@
 
 namespace artf {
 
-/* The tile a workgroup traces: 16 x 16 pixels as four waves of 8 x 8, for coherent neighbours. */
+/* The adjoint's tile: 16 x 16 pixels as four waves of 8 x 8, for coherent neighbours. */
 constexpr int kTile = 16;
 constexpr int kGroup = 256;
+
+/* The tile tracer's tile: 8 x 8 pixels, one wave a workgroup. */
+constexpr int kTraceTile = 8;
+constexpr int kTraceGroup = 64;
 
 /* Waves per SIMD the tracers are built for: 4 for the FP32 and mixed ones (at most 128 VGPRs, which
  * they fit in; the mixed tile kernel measured 10% faster at 4 than at 3); the FP64 one would take 250

@@ -16,8 +16,14 @@ struct Planes {
     float* p[3];
 };
 
+/* Every kernel here is launched with workgroups of one wave. In a frame they run beside the next
+ * frame's trace, whose one-wave workgroups take each wave slot as it frees: a workgroup of several
+ * waves would wait for that many slots free at once on one compute unit, in practice for the trace to
+ * drain, and the next frame's trace would then wait for it. */
+constexpr int kPost = 64;
+
 /* Mean of a pixel's side^2 rays, summed in the order lensTile adds them, times 1/side^2. */
-__global__ __launch_bounds__(256) void resolveKernel(Planes rays, Planes hdr, int pixels, int count, float share) {
+__global__ __launch_bounds__(kPost) void resolveKernel(Planes rays, Planes hdr, int pixels, int count, float share) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
     for (int c = 0; c < 3; ++c) {
@@ -43,34 +49,32 @@ struct Level {
 
 /* The trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, written before this kernel on its
  * stream) summed into settled[0..1] in host memory, with system-scope stores the host sees once the
- * stream is synchronised, and cleared for the next frame's trace. The first workgroup (256 threads) of
- * the kernel that follows the trace calls it, before anything can return. */
+ * stream is synchronised, and cleared for the next frame's trace. The first workgroup (one wave) of the
+ * kernel that follows the trace calls it, before anything can return. */
 struct Settle {
     unsigned long long* counters;
     unsigned long long* settled; /* NULL: nothing to do */
 };
 
 __device__ void settleCounters(const Settle& z) {
+    static_assert(kPost == 64, "the counters are summed by one wave");
     if (!z.settled || blockIdx.x != 0 || blockIdx.y != 0) return;
     unsigned long long* counters = z.counters;
-    __shared__ unsigned long long part[2][4];
     const int t = (int)threadIdx.x;
     for (int half = 0; half < 2; ++half) {
         unsigned long long sum = 0;
-        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += 256) {
+        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += kPost) {
             sum += counters[half * ARTF_HIP_COUNTER_SLOTS + i];
             counters[half * ARTF_HIP_COUNTER_SLOTS + i] = 0ull;
         }
         for (int lanes = 32; lanes > 0; lanes >>= 1) sum += __shfl_xor(sum, lanes);
-        if ((t & 63) == 0) part[half][t >> 6] = sum;
+        if (t == 0) __hip_atomic_store(z.settled + half, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    __syncthreads();
-    if (t < 2) __hip_atomic_store(z.settled + t, part[t][0] + part[t][1] + part[t][2] + part[t][3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 /* Level 0: the mean of 2 x 2 pixels, each first held to what exceeds the threshold, at most the
  * ceiling (software.c:336-349). */
-__global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp,
+__global__ __launch_bounds__(kPost) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp,
                                                         Settle z) {
     stampStart(stamp);
     settleCounters(z);
@@ -93,7 +97,7 @@ __global__ __launch_bounds__(256) void bloomFirstKernel(Planes hdr, Planes bloom
  * that holds it) and their level-0 mean, bloomFirstKernel's, from registers. Ray by ray, the twelve
  * values it needs are loaded together (two pixels of a row as one 8-byte load where the row allows), and
  * all loads come before the first store: a load issued after a store would wait for it. */
-__global__ __launch_bounds__(256) void resolveFirstKernel(Planes rays, Planes hdr, Planes bloom, Level to, int width, int height, int count,
+__global__ __launch_bounds__(kPost) void resolveFirstKernel(Planes rays, Planes hdr, Planes bloom, Level to, int width, int height, int count,
                                                           float share, unsigned long long* stamp, Settle z) {
     stampStart(stamp);
     settleCounters(z);
@@ -155,14 +159,14 @@ __device__ __forceinline__ float upGain(const float* level, const Level& from, i
     return sum * (.25f * ARTF_BLOOM_FADE);
 }
 
-__global__ __launch_bounds__(256) void bloomDownKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+__global__ __launch_bounds__(kPost) void bloomDownKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
     stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
     for (int c = 0; c < 3; ++c) bloom.p[c][to.at + (size_t)y * to.w + x] = downTexel(bloom.p[c] + from.at, from, x, y);
 }
 
-__global__ __launch_bounds__(256) void bloomUpKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+__global__ __launch_bounds__(kPost) void bloomUpKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
     stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
@@ -177,12 +181,11 @@ __global__ __launch_bounds__(256) void bloomUpKernel(Planes bloom, Level from, L
  * arithmetic of bloomDownKernel and bloomUpKernel, and stored: the pyramid the launches level by level
  * leave, in one launch instead of 2 (last - first) - 1. Levels first..last are packed one after
  * another; together they hold at most kSmallFloats / 3 texels, and first+1..last at most kSmallWork
- * (one compute unit does their work). */
-/* 256 threads (a wave per SIMD): the workgroup finds room as soon as one trace workgroup ends, so it
- * runs beside the next frame's trace; 1024 would wait for a whole compute unit to drain. */
-constexpr int kSmallThreads = 256;
+ * (one wave does their work: as every kernel here, it finds room as soon as one trace wave ends, so it
+ * runs beside the next frame's trace). */
+constexpr int kSmallThreads = kPost;
 constexpr int kSmallFloats = 32768; /* 128 KB of the compute unit's 160 KB */
-constexpr int kSmallWork = 4096;
+constexpr int kSmallWork = 1024;
 
 struct Pyramid {
     Level level[ARTF_BLOOM_LEVELS];
@@ -239,11 +242,8 @@ struct ToneStamps {
 
 /* HDR plus the bloom's level 0 (read at half resolution, bilinear, 90% grey), times the exposure, tone
  * mapped, dithered by a hash of the pixel and truncated to bytes: BGRA, alpha 255 (lens.inc:395-440). */
-__global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
-                                                  ToneStamps stamps) {
-    stampStart(stamps.start, stamps.alsoStart);
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= width) return;
+__device__ __forceinline__ void tonePixel(const Planes& hdr, const Planes& bloom, uint32_t* out, const Level& l0, int width, float exposure,
+                                          float spread, int x, int y) {
     const int bw = (int)l0.w, bh = (int)l0.h;
     const float sy = ((float)y + .5f) * .5f - .5f, baseY = floorf(sy), partY = sy - baseY;
     const int y0 = clampInt((int)baseY, 0, bh - 1), y1 = clampInt((int)baseY + 1, 0, bh - 1);
@@ -274,7 +274,18 @@ __global__ __launch_bounds__(256) void toneKernel(Planes hdr, Planes bloom, uint
         pixel |= byte << (16 - 8 * c);
     }
     out[at] = pixel;
-    /* thread 0 of the last workgroup always has a pixel (x < width), so it reaches here */
+}
+
+/* kToneRun pixels of a row a lane, kPost apart (coalesced): fewer, longer workgroups. */
+constexpr int kToneRun = 4;
+
+__global__ __launch_bounds__(kPost) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
+                                                  ToneStamps stamps) {
+    stampStart(stamps.start, stamps.alsoStart);
+    const int y = (int)blockIdx.y;
+    for (int k = 0, x = (int)blockIdx.x * kPost * kToneRun + (int)threadIdx.x; k < kToneRun && x < width; ++k, x += kPost)
+        tonePixel(hdr, bloom, out, l0, width, exposure, spread, x, y);
+    /* the last workgroup's lane 0 starts at a pixel (x < width), so the last row is written once it ends */
     if (stamps.end && blockIdx.x == gridDim.x - 1 && blockIdx.y == gridDim.y - 1 && threadIdx.x == 0)
         __hip_atomic_store(stamps.end, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -287,11 +298,11 @@ static Level levelOf(const ArtfHoleFrame& f, int k) {
 hipError_t launchResolve(const ArtfHoleFrame& f, float* const rays[3], float* const hdr[3], hipStream_t stream) {
     const int pixels = (int)(f.size[0] * f.size[1]), side = (int)f.size[2];
     Planes r = {{rays[0], rays[1], rays[2]}}, h = {{hdr[0], hdr[1], hdr[2]}};
-    hipLaunchKernelGGL(resolveKernel, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, stream, r, h, pixels, side * side, f.image[3]);
+    hipLaunchKernelGGL(resolveKernel, dim3((unsigned)((pixels + kPost - 1) / kPost)), dim3(kPost), 0, stream, r, h, pixels, side * side, f.image[3]);
     return hipGetLastError();
 }
 
-static dim3 rowsOf(const Level& l) { return dim3((l.w + 255) / 256, l.h); }
+static dim3 rowsOf(const Level& l) { return dim3((l.w + kPost - 1) / kPost, l.h); }
 
 static uint32_t texelsFrom(const ArtfHoleFrame& f, int k) {
     const int last = (int)f.size[3] - 1;
@@ -320,7 +331,7 @@ static bool bloomRest(const ArtfHoleFrame& f, const Planes& b, hipStream_t strea
     };
     for (int k = 1; k <= small; ++k) {
         const Level to = levelOf(f, k);
-        hipLaunchKernelGGL(bloomDownKernel, rowsOf(to), dim3(256), 0, stream, b, levelOf(f, k - 1), to, first());
+        hipLaunchKernelGGL(bloomDownKernel, rowsOf(to), dim3(kPost), 0, stream, b, levelOf(f, k - 1), to, first());
     }
     if (small < last) {
         Pyramid p = {};
@@ -329,7 +340,7 @@ static bool bloomRest(const ArtfHoleFrame& f, const Planes& b, hipStream_t strea
     }
     for (int k = small < last ? small : last - 1; k >= 0; --k) {
         const Level to = levelOf(f, k);
-        hipLaunchKernelGGL(bloomUpKernel, rowsOf(to), dim3(256), 0, stream, b, levelOf(f, k + 1), to, first());
+        hipLaunchKernelGGL(bloomUpKernel, rowsOf(to), dim3(kPost), 0, stream, b, levelOf(f, k + 1), to, first());
     }
     return any;
 }
@@ -339,7 +350,7 @@ hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const
     if (levels < 1 || levels > ARTF_BLOOM_LEVELS) return hipErrorInvalidValue;
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
     const Level l0 = levelOf(f, 0);
-    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr,
+    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr,
                        Settle{});
     bloomRest(f, b, stream, nullptr);
     return hipGetLastError();
@@ -347,11 +358,11 @@ hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const
 
 static float spreadOf(const ArtfHoleFrame& f) { return f.look[3] * (1.f - ARTF_BLOOM_FADE); }
 
-static dim3 pixelRows(const ArtfHoleFrame& f) { return dim3((f.size[0] + 255) / 256, f.size[1]); }
+static dim3 pixelRows(const ArtfHoleFrame& f) { return dim3((f.size[0] + kPost * kToneRun - 1) / (kPost * kToneRun), f.size[1]); }
 
 hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* const bloom[3], uint32_t* bgra, hipStream_t stream) {
     Planes h = {{hdr[0], hdr[1], hdr[2]}}, b = {{bloom[0], bloom[1], bloom[2]}};
-    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, stream, h, b, bgra, levelOf(f, 0), (int)f.size[0], f.image[0], spreadOf(f), ToneStamps{});
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(kPost), 0, stream, h, b, bgra, levelOf(f, 0), (int)f.size[0], f.image[0], spreadOf(f), ToneStamps{});
     return hipGetLastError();
 }
 
@@ -375,13 +386,13 @@ hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
             return e;
     }
     if (side > 1)
-        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(256), 0, post, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
+        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(kPost), 0, post, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
                            r ? r + 3 : nullptr, z);
     else
-        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(256), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
+        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
     const bool rest = bloomRest(f, b, post, r ? r + 4 : nullptr);
     const ToneStamps ts = r ? ToneStamps{r + 5, rest ? nullptr : r + 4, r + 6} : ToneStamps{};
-    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(256), 0, post, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), ts);
+    hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(kPost), 0, post, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), ts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (a.postStream && a.done) return hipEventRecord((hipEvent_t)a.done, post);
     return hipSuccess;

@@ -1,7 +1,8 @@
-/* artful-hip — the tracing kernel: a workgroup of 256 lanes traces a 16 x 16 tile of pixels (four
- * waves of 8 x 8, so neighbouring rays, which take similar paths, share a wave) for one of the
- * pixel's side^2 rays (grid z). Included by trace_exact.hip and trace_fast.hip, which differ only in
- * their contraction rule. */
+/* artful-hip — the tracing kernel: a workgroup of one wave traces an 8 x 8 tile of pixels (neighbouring
+ * rays, which take similar paths, share the wave) for one of the pixel's side^2 rays (grid z). One wave
+ * a workgroup because rays end at their own step: a wave's slot on its SIMD is taken by the next tile as
+ * soon as the wave ends, where the waves of a larger workgroup hold their slots until its slowest wave
+ * ends. Included by trace_exact.hip and trace_fast.hip, which differ only in their contraction rule. */
 #pragma once
 
 #include "launch.h"
@@ -61,14 +62,14 @@ __device__ __forceinline__ void countSteps(unsigned long long* counters, int ste
 }
 
 template <int Precision>
-__global__ __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
+__global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
     stampStart(a.startStamp);
     const ArtfHoleFrame& f = a.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
-    const int py = a.rowBegin + blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+    const int lane = (int)threadIdx.x;
+    const int px = (int)blockIdx.x * kTraceTile + (lane & 7);
+    const int py = a.rowBegin + (int)blockIdx.y * kTraceTile + (lane >> 3);
     const int sub = blockIdx.z, sx = sub % side, sy = sub / side;
     const bool valid = px < width && py < rowEnd;
 
@@ -114,8 +115,8 @@ inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHi
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
     const int rows = rowEnd - a.rowBegin;
     if (width <= 0 || rows <= 0 || a.rowBegin < 0 || side < 1 || side > 8) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((rows + kTile - 1) / kTile), (unsigned)(side * side));
-    hipLaunchKernelGGL(kernel, grid, dim3(kGroup), 0, stream, a);
+    const dim3 grid((unsigned)((width + kTraceTile - 1) / kTraceTile), (unsigned)((rows + kTraceTile - 1) / kTraceTile), (unsigned)(side * side));
+    hipLaunchKernelGGL(kernel, grid, dim3(kTraceGroup), 0, stream, a);
     return hipGetLastError();
 }
 

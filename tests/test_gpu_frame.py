@@ -62,17 +62,17 @@ def dev(gpu):
 
 def small_first(f):
     """Where the one-workgroup pass starts: the first level k whose levels k..last fit 32768 floats of
-    LDS and whose levels k+1..last hold at most 4096 texels; the last level when none does."""
+    LDS and whose levels k+1..last hold at most 1024 texels; the last level when none does."""
     last = f.size[3] - 1
     end = f.bloom[last][0] + f.bloom[last][1] * f.bloom[last][2]
     for k in range(last):
-        if (end - f.bloom[k][0]) * 3 <= 32768 and end - f.bloom[k + 1][0] <= 4096:
+        if (end - f.bloom[k][0]) * 3 <= 32768 and end - f.bloom[k + 1][0] <= 1024:
             return k
     return last
 
 
-PYRAMIDS = [(48, 27, 8, 0), (320, 200, 8, 1), (640, 400, 8, 2), (1920, 1080, 8, 3), (3840, 2160, 8, 4), (1920, 1080, 3, 2),
-            (1920, 1080, 2, 1), (1920, 1080, 1, 0), (333, 77, 8, 0), (1001, 3, 8, 0), (1, 1, 8, 0), (5, 3, 4, 0)]
+PYRAMIDS = [(48, 27, 8, 0), (320, 200, 8, 2), (640, 400, 8, 3), (1920, 1080, 8, 4), (3840, 2160, 8, 5), (1920, 1080, 3, 2),
+            (1920, 1080, 2, 1), (1920, 1080, 1, 0), (333, 77, 8, 1), (1001, 3, 8, 0), (1, 1, 8, 0), (5, 3, 4, 0)]
 
 
 @pytest.mark.parametrize("w,h,levels,first", PYRAMIDS)
