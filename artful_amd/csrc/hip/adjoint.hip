@@ -1,19 +1,20 @@
 /* artful-hip — the adjoint kernel: each lane runs one ray forward and back (adjoint.h), and each
- * workgroup sums its rays' gradients into 32 doubles (a fixed order: the sums are the same every run).
- * Built without contraction, so its forward light equals trace_exact.hip's analytic-disc light. */
+ * workgroup (one wave over an 8 x 8 tile, as the tile tracer) sums its rays' gradients into 32 doubles
+ * (a fixed order: the sums are the same every run). Built without contraction, so its forward light
+ * equals trace_exact.hip's analytic-disc light. */
 #include "adjoint.h"
 #include "trace_kernel.h"
 
 namespace artf {
 
 template <typename R>
-__global__ __launch_bounds__(kGroup) void adjointKernel(ArtfHipAdjointArgs args) {
+__global__ __launch_bounds__(kTraceGroup) void adjointKernel(ArtfHipAdjointArgs args) {
     const ArtfHipTraceArgs& a = args.trace;
     const ArtfHoleFrame& f = a.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int px = blockIdx.x * kTile + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * kTile + (wave >> 1) * 8 + (lane >> 3);
+    const int lane = (int)threadIdx.x;
+    const int px = (int)blockIdx.x * kTraceTile + (lane & 7);
+    const int py = (int)blockIdx.y * kTraceTile + (lane >> 3);
     const int sub = blockIdx.z, sx = sub % side, sy = sub / side;
     const bool valid = px < width && py < height;
 
@@ -61,33 +62,28 @@ __global__ __launch_bounds__(kGroup) void adjointKernel(ArtfHipAdjointArgs args)
         a.out[1][at] = light[1];
         a.out[2][at] = light[2];
     }
-    /* the workgroup's sum, in a fixed order: across each wave, then the four waves */
-    __shared__ double waves[4][32];
+    /* the wave's sum, in a fixed order (the same butterfly on every lane), stored by lane k */
+    const size_t block = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    double mine = 0.0;
     for (int k = 0; k < ADJ_COUNT; ++k) {
         double v = grad[k];
         for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset);
-        if (lane == 0) waves[wave][k] = v;
+        if (lane == k) mine = v;
     }
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        const int k = threadIdx.x;
-        const double sum = k < ADJ_COUNT ? ((waves[0][k] + waves[1][k]) + (waves[2][k] + waves[3][k])) : 0.0;
-        const size_t block = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-        args.partials[block * 32 + k] = sum;
-    }
+    if (lane < 32) args.partials[block * 32 + lane] = mine;
 }
 
 int adjointBlocks(const ArtfHoleFrame& f) {
     const int side = (int)f.size[2];
-    return (int)(((f.size[0] + kTile - 1) / kTile) * ((f.size[1] + kTile - 1) / kTile)) * side * side;
+    return (int)(((f.size[0] + kTraceTile - 1) / kTraceTile) * ((f.size[1] + kTraceTile - 1) / kTraceTile)) * side * side;
 }
 
 hipError_t launchAdjoint(const ArtfHipAdjointArgs& args, hipStream_t stream) {
     const ArtfHoleFrame& f = args.trace.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
-    const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((height + kTile - 1) / kTile), (unsigned)(side * side));
-    if (args.trace.precision == ARTF_PRECISION_FP64) hipLaunchKernelGGL(adjointKernel<double>, grid, dim3(kGroup), 0, stream, args);
-    else hipLaunchKernelGGL(adjointKernel<float>, grid, dim3(kGroup), 0, stream, args);
+    const dim3 grid((unsigned)((width + kTraceTile - 1) / kTraceTile), (unsigned)((height + kTraceTile - 1) / kTraceTile), (unsigned)(side * side));
+    if (args.trace.precision == ARTF_PRECISION_FP64) hipLaunchKernelGGL(adjointKernel<double>, grid, dim3(kTraceGroup), 0, stream, args);
+    else hipLaunchKernelGGL(adjointKernel<float>, grid, dim3(kTraceGroup), 0, stream, args);
     return hipGetLastError();
 }
 

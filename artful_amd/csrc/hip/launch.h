@@ -8,11 +8,7 @@
 
 namespace artf {
 
-/* The adjoint's tile: 16 x 16 pixels as four waves of 8 x 8, for coherent neighbours. */
-constexpr int kTile = 16;
-constexpr int kGroup = 256;
-
-/* The tile tracer's tile: 8 x 8 pixels, one wave a workgroup. */
+/* The tile of the tile tracer and the adjoint: 8 x 8 pixels, one wave a workgroup. */
 constexpr int kTraceTile = 8;
 constexpr int kTraceGroup = 64;
 
