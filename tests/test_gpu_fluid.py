@@ -90,3 +90,50 @@ def test_renderer_with_the_disc_on_the_device(gpu):
             assert np.array_equal(r.read_pixels(), want_px), t
     finally:
         r.close(), host.close(), dev.close()
+
+
+def test_queued_frames_with_the_disc_on_the_device(gpu):
+    """Frames of a scene whose fluid steps on the GPU, queued back to back with no wait between them, each
+    on a renderer of its own (so each frame's HDR stays to be read afterwards): every fluid step writes the
+    field buffer no queued frame reads, and every frame reads the field of its own moment, so each frame
+    equals the host scene's frame at that moment bit for bit. Then ten renderers read one field, more
+    than the fluid keeps read marks for, before the fluid steps twice and is read again."""
+    desc, cam = cases.make(cases.CASES["advance"])
+    host, dev = gpu.Scene(desc, cam), gpu.Scene(cam=cam)
+    dev.set_disc_on_device(0)
+    dev.set_hole(desc)
+    times = [round(.2 * k, 1) for k in range(1, 11)]
+    rs = [gpu.Renderer(96, 60) for _ in times]
+    ref = gpu.Renderer(96, 60)
+    try:
+        for r in rs:  # buffers made and nothing left queued before the timed part
+            assert r.draw(dev)
+            r.read_hdr()
+        for r, t in zip(rs, times):
+            desc.time = t
+            dev.set_hole(desc)
+            assert r.draw(dev)
+        got = [r.read_hdr() for r in rs]
+        for k, t in enumerate(times):
+            desc.time = t
+            host.set_hole(desc)
+            assert ref.draw(host)
+            assert np.array_equal(got[k].view(np.uint32), ref.read_hdr().view(np.uint32)), t
+        for r in rs:  # ten reads of the field of one moment
+            assert r.draw(dev)
+        for t in (times[-1] + .2, times[-1] + .4):
+            desc.time = t
+            dev.set_hole(desc)
+        assert rs[0].draw(dev)
+        want_same = ref.read_hdr()  # the host frame at times[-1]
+        for r in rs[1:]:
+            assert np.array_equal(r.read_hdr().view(np.uint32), want_same.view(np.uint32))
+        for t in (times[-1] + .2, times[-1] + .4):
+            desc.time = t
+            host.set_hole(desc)
+        assert ref.draw(host)
+        assert np.array_equal(rs[0].read_hdr().view(np.uint32), ref.read_hdr().view(np.uint32))
+    finally:
+        for r in rs:
+            r.close()
+        ref.close(), host.close(), dev.close()
