@@ -98,7 +98,9 @@ static bool eddyRoom(ArtfDeviceFluid* f, int n) {
     return f->eddies != NULL;
 }
 
-/* steps steps: their eddies drawn in order on the host, sent at once, then the passes. */
+/* steps steps: their eddies drawn in order on the host, sent at once, then the passes. eddiesHost is
+ * pageable memory: a copy from it has read it when hipMemcpyAsync returns (the runtime stages pageable
+ * sources), so the next call may draw into it while this one's passes are still queued. */
 static bool run(ArtfDeviceFluid* f, int steps) {
     const int n = steps * ARTF_FLUID_STEP_EDDIES;
     if (!eddyRoom(f, n)) return false;

@@ -204,8 +204,8 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s, int set) {
         h->mapVersion = hole->mapVersion;
     }
     if (hole->deviceFluid && hole->discDevice == h->device) {
-        /* the scene steps its fluid on this device: read it where it is (its steps are done when the
-         * scene's call returns) */
+        /* the scene steps its fluid on this device: read it where it is (the draw orders its trace after
+         * the current field's write and marks its read, artfDeviceFluidReadOn/ReadDone) */
         h->fluidScene = 0;
         return true;
     }
@@ -219,6 +219,8 @@ static bool syncScene(HipRenderer* h, const ArtfScene* s, int set) {
             if (!status) status = artfHipStreamSync(stream);
             artfFree(field);
         } else {
+            /* from the scene's pageable field: the copy has read it when this returns (the runtime
+             * stages pageable sources), so the scene's next fluid step may rewrite it at once */
             status = artfHipUpload(h->fluid, artfFluidField(hole->fluid), bytes, stream);
         }
         if (!ok(h, status, "fluid upload")) return false;
