@@ -22,7 +22,8 @@ int artfHipDeviceInfo(int device, char* name, size_t size, int* computeUnits) {
     hipDeviceProp_t p;
     const hipError_t e = hipGetDeviceProperties(&p, device);
     if (e != hipSuccess) return (int)e;
-    if (name && size) snprintf(name, size, "%s (%s)", p.name, p.gcnArchName);
+    /* (the marketing name can be empty, as under some drivers: then the architecture alone) */
+    if (name && size) snprintf(name, size, p.name[0] ? "%s (%s)" : "%s%s", p.name, p.gcnArchName);
     if (computeUnits) *computeUnits = p.multiProcessorCount;
     return 0;
 }
