@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 
 WIDTH, HEIGHT, SAMPLES = 1920, 1080, 2
 SPIN, RADIUS, INCLINATION, OUTER = .998, 60.0, 60.0, 30.0
-FLOPS_PER_STEP = {True: 147, False: 122}  # per Nystrom step, spin != 0 / spin == 0 (count rule in DESIGN.md)
+FLOPS_PER_STEP = {True: 147, False: 122}  # per Nystrom step, spin != 0 / spin == 0 (+ - x / one each, an FMA two)
 PEAK_FP32_TFLOPS = 157.3                    # MI355X vector FP32, published
 PEAK_HBM_GBS = 8000.0
 
@@ -178,7 +178,7 @@ def cpu_baseline(native, seconds=15.0):
 
 
 def adjoint_fit(native, iterations=200):
-    """512x512, one ray a pixel, the analytic disc; the adjoint's cost against the
+    """The adjoint workload: 512x512, one ray a pixel, the analytic disc; the adjoint's cost against the
     forward trace, and a 200-step Adam fit of (spin, inclination, disc inner, outer, q) from the start
     point to the image of the hidden parameters."""
     from tests import kerr_autograd as K
@@ -246,7 +246,7 @@ def per_view_tiles(native, views, trc, reps=3):
 
 
 def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=None):
-    """64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
+    """The batch workload: 64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
     U[5,85], eye r U[20,100], disc outer U[10,40], each with its own rings and fluid; rays of up to 5000
     steps; one persistent launch replayed from a HIP graph, schedule from the tuner. Beside it, the
     analytic disc through the same batch and through the tile kernel one launch per view."""
@@ -346,7 +346,7 @@ def animation(native, frames=100, warmup=50):
 
 
 def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, draws=3):
-    """SURVEY.md (f)4's huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays, 6.4 GB of ray planes)
+    """The huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays, 6.4 GB of ray planes)
     as a batch of one view in slices of 2^20 tiles (67 M rays, ~8 ms a launch: watchdog-sized), beside
     the renderer's single launch of the tile kernel."""
     desc, cam = workload_scene(native, samples=side)
@@ -379,7 +379,7 @@ def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, dr
 
 
 def precision_study(native, width=3840, height=2160):
-    """3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
+    """The precision study: 3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
     75, 89}, eye at r 50, painted disc ISCO..20, Doppler 1: FP32, mixed and FP64 traces, each image's
     relative HDR error against its FP64 trace, and rays/s."""
     from artful_amd.workloads import view_of
@@ -433,8 +433,8 @@ def main():
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build runs the tracer on one device (multi-GPU sharding is a design "
-              "in DESIGN.md, not a measured path). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the tracer runs on one device (sharding a frame by row bands or a batch by "
+              "views across GPUs is not built). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from artful_amd import native

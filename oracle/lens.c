@@ -1,5 +1,5 @@
-/* The oracle's tracer, bloom and tone map (oracle.h). Test infrastructure: the checker of the HIP
- * path, never a product path. Built by oracle/build.py with -O2 -ffp-contract=off -fno-math-errno
+/* The oracle's tracer, bloom and tone map (oracle.h). Test infrastructure: what the HIP path is
+ * tested against, never a product path. Built by oracle/build.py with -O2 -ffp-contract=off -fno-math-errno
  * -fno-trapping-math, the reference's rules (CMakeLists.txt:54-58). */
 #define _GNU_SOURCE
 #include "oracle.h"

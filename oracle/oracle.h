@@ -1,5 +1,5 @@
-/* The oracle: a plain C11, scalar, CPU restatement of Artful 0.4.0's black-hole path, used only as
- * the checker of this project's HIP path (tests, smoke check, the bench's CPU baseline). It is test
+/* The oracle: a plain C11, scalar, CPU restatement of Artful 0.4.0's black-hole path, used only to
+ * test the HIP path against (tests, the smoke check, the bench's CPU baseline). It is test
  * infrastructure and never a product path.
  *
  *   oracleTrace    src/software/lens.inc:99-383 (lensTrace + lensTile), FP32 bit for bit with the

@@ -1,6 +1,6 @@
 """ctypes binding of the oracle (oracle/liboracle.so) and, where it was built, of the reference
 library (oracle/_ref/libartful_ref.so). Test infrastructure only: imported by tests/, by
-__graft_entry__.smoke() and by bench.py's CPU-baseline leg, as the checker."""
+__graft_entry__.smoke() and by bench.py's CPU-baseline leg, to test the HIP path against."""
 from __future__ import annotations
 
 import ctypes as C

@@ -16,7 +16,7 @@ sys.path.insert(0, str(ROOT))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (run with -m gpu)")
 
 
 @pytest.fixture(scope="session")

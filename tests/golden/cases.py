@@ -32,10 +32,10 @@ def _grid():
 
 
 CASES = {
-    # 320x180, spin .9, r 50, inclination 80, fovY .35, disc ISCO..20, Doppler 1
+    # the CPU-sized view: 320x180, spin .9, r 50, inclination 80, fovY .35, disc ISCO..20, Doppler 1
     "config1": dict(distance=50.0, tilt=math.radians(10), fov=.35, desc=dict(spin=.9, discOuter=20.0, doppler=1.0),
                     sizes=[(320, 180), (64, 36)], store=(64, 36)),
-    # BASELINE config 2's view at a tenth of the size: spin .998, r 60, inclination 60, disc ISCO..30, 2x2 rays
+    # the 1080p workload's view at a tenth of the size: spin .998, r 60, inclination 60, disc ISCO..30, 2x2 rays
     "config2_small": dict(distance=60.0, tilt=math.radians(30), fov=.35, desc=dict(spin=.998, discOuter=30.0, doppler=1.0, samples=2),
                           sizes=[(192, 108)]),
     # tests.c:4317-4340: spin .9, bloom 1, 2x2 rays, shutter 3, a moving eye

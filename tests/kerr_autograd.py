@@ -1,7 +1,7 @@
 """A test harness: the differentiable analytic-disc renderer (include/artful/adjoint.h, through ctypes)
 as a torch.autograd.Function of (spin, inclination in degrees, disc inner, disc outer, q).
 
-The scene is BASELINE config 5's: 512x512 by default, one ray a pixel, eye at r = 50 looking at the
+The scene is the adjoint workload's: 512x512 by default, one ray a pixel, eye at r = 50 looking at the
 hole, thin analytic disc (emissivity (inner/r)^q), Doppler 1, no stars, no bloom; 2000 steps, escape
 at r > 100. torch is used here as the optimiser's plumbing only: the gradients come from the HIP
 adjoint kernel."""

@@ -2,7 +2,7 @@
 GPU and on the CPU oracle; its FP32 form against its FP64 form; its forward image against the tracer's;
 the torch.autograd.Function harness; a short fit; and its cost against the forward pass.
 
-Loss: MSE of the float32 HDR against a target rendered with hidden parameters (BASELINE config 5's
+Loss: MSE of the float32 HDR against a target rendered with hidden parameters (the adjoint workload's
 scene, smaller). Parameters: spin, inclination (degrees), disc inner, disc outer, q.
 
 A pixel whose ray changes its fate between theta - h and theta + h (captured or not, one disc crossing
@@ -161,7 +161,7 @@ def test_fit_moves_toward_hidden(ctx, problem):
 
 
 def test_adjoint_costs_at_most_four_forwards(gpu):
-    """BASELINE config 5 at full size: 512x512, one ray a pixel."""
+    """The adjoint workload at full size: 512x512, one ray a pixel."""
     c = gpu.HoleAdjoint(512, 512)
     target = K.render(c, K.HIDDEN)
     ratios = []

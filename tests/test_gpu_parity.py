@@ -95,7 +95,7 @@ def test_mixed_closer_to_fp64_than_fp32(renderer, scene_of, oracle):
 
 @pytest.mark.parametrize("precision", ["fp32", "fp32_fast", "mixed", "fp64"])
 def test_analytic_disc(renderer, native, oracle, precision):
-    """The analytic r^-q disc (BASELINE config 1's extension: 2000 steps, escape at r > 100) against the
+    """The analytic r^-q disc on the CPU-sized view (2000 steps, escape at r > 100) against the
     FP64 oracle: the smooth disc keeps every precision within 1e-3 on nearly all pixels."""
     desc, cam = cases.make(cases.CASES["config1"])
     desc.stars = 0.0
