@@ -345,6 +345,44 @@ def animation(native, frames=100, warmup=50):
     return out
 
 
+EXTENDED_LAW = dict(disc="analytic", emissivity=3.0, jitter=2, bloom_levels=5)
+
+
+def extended_form(native, renderer=None, scene=None, steps=100, warmup=30):
+    """The 1080p workload with its extended ray law: 2x2 rays a pixel at places jittered from seed 2
+    inside their strata, the analytic disc (emissivity (inner/r)^3), 5 bloom levels. FP32 and mixed
+    frames back to back, and each one's relative HDR error against the FP64 frame of the same law."""
+    own = renderer is None
+    if own:
+        desc, cam = workload_scene(native)
+        scene = native.Scene(desc, cam)
+        renderer = native.Renderer(WIDTH, HEIGHT)
+    out = {"law": dict(EXTENDED_LAW)}
+    renderer.set_tracing("fp64", **EXTENDED_LAW)
+    if not renderer.draw(scene):
+        raise RuntimeError("draw failed")
+    ref64 = renderer.read_hdr().astype(np.float64)
+    for p in ("fp32", "mixed"):
+        renderer.set_tracing(p, **EXTENDED_LAW)
+        for _ in range(warmup):
+            if not renderer.draw(scene):
+                raise RuntimeError("draw failed")
+        renderer.hole_stats()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if not renderer.draw(scene):
+                raise RuntimeError("draw failed")
+        st = renderer.hole_stats()  # waits for the last frame
+        wall = time.perf_counter() - t0
+        out[p] = {"ms_per_frame": wall / steps * 1e3, "rays_per_s": st.rays * steps / wall, "bloom_levels": st.bloomLevels,
+                  "error_vs_fp64": rel_error(renderer.read_hdr(), ref64)}
+    renderer.set_tracing("fp32")
+    if own:
+        renderer.close()
+        scene.close()
+    return out
+
+
 def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, draws=3):
     """The huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays, 6.4 GB of ray planes)
     as a batch of one view in slices of 2^20 tiles (67 M rays, ~8 ms a launch: watchdog-sized), beside
@@ -427,7 +465,7 @@ def main():
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames and the FP32 kernel timing (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
-    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge"], help="run one extra leg alone and print its JSON")
+    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge", "extended"], help="run one extra leg alone and print its JSON")
     ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
@@ -452,7 +490,7 @@ def main():
     if args.only:
         sched = tuple(int(v) for v in args.batch_schedule.split(",")) if args.batch_schedule else None
         legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit,
-                "animation": animation, "huge": huge_frame}
+                "animation": animation, "huge": huge_frame, "extended": extended_form}
         leg = legs[args.only](native)
         print(json.dumps({args.only: leg}), flush=True)
         return
@@ -485,6 +523,9 @@ def main():
     ref64 = hdrs["fp64"].astype(np.float64)
     errors = {p: rel_error(h, ref64) for p, h in (("fp32", hdr32), ("fp32_fast", hdrs["fp32_fast"]), ("mixed", hdrs["mixed"]))}
 
+    extended = extended_form(native, renderer, scene)
+    log(f"extended: {extended}")
+
     flops = fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0]
     achieved = flops / (kernel["fp32"] * 1e-3) / 1e12
     traffic = traffic_from_profiles()
@@ -515,6 +556,7 @@ def main():
         "frame_stages_ms": {"trace": fp32["trace_ms"], "post": fp32["post_ms"]},
         "steps_per_ray": fp32["steps_per_frame"] / rays, "active_lane_fraction": fp32["active_lanes"],
         "rel_error_vs_fp64": errors,
+        "extended_form": extended,
         "adjoint_over_forward": adjoint["adjoint_over_forward"],
         "adjoint": adjoint,
         "animation": anim,
