@@ -130,9 +130,11 @@ static hipStream_t fresh(void* stream) {
     return (hipStream_t)stream;
 }
 
+/* A frame the kernels can take: at most 65535 rows (the resolve and tone map launch a row of
+ * workgroups per row of pixels, and a grid is at most 65535 workgroups high), fewer than 2^31 rays. */
 static bool frameFits(const ArtfHoleFrame* f) {
-    return f && f->size[0] > 0 && f->size[1] > 0 && f->size[2] >= 1 && f->size[2] <= 8 && f->size[3] >= 1 && f->size[3] <= ARTF_BLOOM_LEVELS &&
-           (uint64_t)f->size[0] * f->size[1] * f->size[2] * f->size[2] < (1ull << 31);
+    return f && f->size[0] > 0 && f->size[1] > 0 && f->size[1] <= 65535 && f->size[2] >= 1 && f->size[2] <= 8 && f->size[3] >= 1 &&
+           f->size[3] <= ARTF_BLOOM_LEVELS && (uint64_t)f->size[0] * f->size[1] * f->size[2] * f->size[2] < (1ull << 31);
 }
 
 static bool traceFits(const ArtfHipTraceArgs* a) {

@@ -64,7 +64,10 @@ const char* artfHipErrorText(int status);
 #define ARTF_HIP_COUNTER_SLOTS 1024
 
 typedef struct ArtfHipTraceArgs {
-    ArtfHoleFrame frame;     /* from artfHoleFrame: FP32 uniforms */
+    /* from artfHoleFrame: FP32 uniforms. artfHipTrace, artfHipResolve, artfHipBloom, artfHipToneMap,
+     * artfHipDrawFrame and artfHipAdjoint take frames of at most 65535 rows and fewer than 2^31 rays
+     * (side^2 * width * height), and refuse others with hipErrorInvalidValue, launching nothing. */
+    ArtfHoleFrame frame;
     ArtfHoleFrame64 frame64; /* from artfHoleFrame64: read by the MIXED and FP64 tracers */
     const float* map;        /* device, ARTF_HOLE_MAP_FLOATS */
     const float* fluid;      /* device, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT */

@@ -58,6 +58,13 @@ def test_trace_rejects_bad_arguments(native):
     assert L.artfHipTrace(C.byref(a), None) != 0          # no buffers
     a.frame.size[2] = 9
     assert L.artfHipTrace(C.byref(a), None) != 0          # side beyond 8
+    a.frame.size[2] = 1
+    a.map = a.fluid = 1
+    a.out[0] = a.out[1] = a.out[2] = 1
+    a.frame.size[0], a.frame.size[1] = 1, 65536
+    assert L.artfHipTrace(C.byref(a), None) != 0          # taller than a grid can be
+    a.frame.size[0], a.frame.size[1], a.frame.size[2] = 65536, 65535, 8
+    assert L.artfHipTrace(C.byref(a), None) != 0          # 2^31 rays or more
     f = abi.HoleFrame()
     f.size[0], f.size[1], f.size[2], f.size[3] = 16, 16, 1, 1
     P3 = C.c_void_p * 3
