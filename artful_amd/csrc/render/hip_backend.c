@@ -248,6 +248,12 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
     artfHoleFrameLevels(scene, h->width, h->height, h->tracing.bloomLevels, &h->frame, &bloomPixels);
     const int side = (int)h->frame.size[2];
     const size_t rays = (size_t)h->width * h->height * side * side;
+    if (h->height > 65535 || rays >= ((size_t)1 << 31)) {
+        /* beyond a launch grid (the renderer stays usable: a smaller size draws) */
+        artfLog("artful: HIP: a %dx%d frame at %d rays a pixel is beyond the kernels' grids (at most 65535 rows, under 2^31 rays)",
+                h->width, h->height, side * side);
+        return false;
+    }
     if (!ensure(h, bloomPixels, rays)) return false;
     /* the other set than the last frame's, once the frame before last is done with it */
     const int set = h->set ^ 1;

@@ -204,6 +204,19 @@ def test_device_lost_contract(gpu, native, scene_of):
     r2.close()
 
 
+def test_frame_beyond_the_grid_is_refused(gpu, scene_of):
+    """A frame taller than a launch grid (65535 rows) is refused on the host, nothing launched; the
+    renderer then draws at a size that fits."""
+    s = scene_of("config1")
+    r = gpu.Renderer(2, 65536)
+    try:
+        assert not r.draw(s)
+        r.resize(32, 18)
+        assert r.draw(s)
+    finally:
+        r.close()
+
+
 def test_no_hole_and_overlay(renderer, native):
     s = native.Scene()
     renderer.resize(8, 4)
