@@ -6,7 +6,10 @@
  *     the frame worked out in doubles; the FP32 coefficients are these, rounded once;
  *   - the radial and polar turning points: a step in which u' (w) or m' (v.z) changes sign is taken
  *     again in FP64 from the FP32 state, widened;
- *   - the steps near the horizon, where u passes .9 of its capture value.
+ *   - the step next to the horizon: a step whose FP32 result has u past its capture value is taken
+ *     again in FP64, which decides whether the ray is captured there. (FP64 for every step with u
+ *     past .9 of the capture value as well costs 6% more and changes one image of the 4K study in 16,
+ *     by 2% of its mean error: a captured ray's light is what it gathered before.)
  * The FP32 state after such a step is the FP64 result, rounded. Everything else, the disc-plane
  * crossings, the escape and the light they give included, is the FP32 arithmetic of tracer.h (as
  * the FP32-with-FMA tracer does it). Taking the crossings and the escape in FP64 as well gave a third
@@ -50,14 +53,13 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
     bool sky = false;
     float awayX = 0, awayY = 0, awayZ = 0, awayPsi = 0;
     const float uHorizon = (float)e.uHorizon, uEscape = law.uEscape;
-    const float uNear = (float)(.9 * e.uHorizon);
     int steps = law.maxSteps;
 #pragma unroll 2 /* two steps a trip round the loop, as traceRay: 3.6% */
     for (int i = 0; i < law.maxSteps; ++i) {
         RayState<float> n;
         StepLooks<float> looks;
         nystromStep(cf, s, n, looks);
-        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uNear || s.u > uNear) { /* this step again, in FP64 */
+        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uHorizon) { /* this step again, in FP64 */
             const RayState<double> w = castState<double>(s);
             RayState<double> m;
             StepLooks<double> l;

@@ -153,15 +153,15 @@ struct MixedLane {
         acc.r = acc.g = acc.b = 0.f, acc.through = 1.f;
     }
 
-    /* traceRayMixed's step: FP64 again at turning points and near the horizon, the rest in FP32 */
+    /* traceRayMixed's step: FP64 again at turning points and for the step that reaches the horizon, the
+     * rest in FP32 */
     __device__ __forceinline__ bool step(const WaveView& w, float time, bool& sky) {
         const ArtfHipView& v = *w.v;
         const float uHorizon = (float)v.frame64.spin[1], uEscape = w.uEscape;
-        const float uNear = (float)(.9 * v.frame64.spin[1]);
         RayState<float> n;
         StepLooks<float> looks;
         nystromStep(cf, s, n, looks);
-        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uNear || s.u > uNear) {
+        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uHorizon) {
             const RayState<double> x = castState<double>(s);
             RayState<double> m;
             StepLooks<double> l;

@@ -77,7 +77,7 @@ def test_precisions_against_oracle64(renderer, scene_of, oracle, precision, mean
 
 
 def test_mixed_closer_to_fp64_than_fp32(renderer, scene_of, oracle):
-    """The mixed tracer (FP64 at the setup, the turning points and near the horizon) exists to be nearer
+    """The mixed tracer (FP64 at the setup, the turning points and the step next to the horizon) exists to be nearer
     the FP64 result than the FP32 tracer: its mean relative error and its share of pixels past 1e-3
     against the FP64 oracle are the smaller, on two views."""
     for name, (w, h) in (("config1", (320, 180)), ("config2_small", (192, 108))):
