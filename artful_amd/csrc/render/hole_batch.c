@@ -30,12 +30,15 @@ struct ArtfHoleBatch {
     BatchView* views;
     void* stream;
     void* events[2];
+    void* side;               /* the second stream of a draw in slices */
+    void* order[2];           /* fork and join of the two streams */
     float* map;               /* device, ARTF_HOLE_MAP_FLOATS */
     float* rings;             /* device, count * RING_FLOATS */
     float* fluids;            /* device, count * FLUID_FLOATS, painted disc only */
     ArtfHipView* deviceViews; /* device, count */
     uint32_t* firstTile;      /* device, count + 1 */
-    void* workspace;          /* device: queue, counters, stamps */
+    void* workspace;          /* device: two sets of queues, counters, stamps */
+    uint32_t* queues[2];      /* in the workspace: a set of queues for each stream */
     ArtfHipBatchArgs args;
     unsigned long long* countersHost;
     unsigned long long* stampsHost;
@@ -67,8 +70,10 @@ void artfHoleBatchFree(ArtfHoleBatch* b) {
     artfDeviceMapFree(b->map), artfHipFree(b->rings), artfHipFree(b->fluids);
     artfHipFree(b->deviceViews), artfHipFree(b->firstTile), artfHipFree(b->workspace);
     artfFree(b->countersHost), artfFree(b->stampsHost);
-    for (int i = 0; i < 2; ++i) artfHipEventDestroy(b->events[i]);
+    if (b->side) artfHipStreamSync(b->side);
+    for (int i = 0; i < 2; ++i) artfHipEventDestroy(b->events[i]), artfHipEventDestroy(b->order[i]);
     artfHipStreamDestroy(b->stream);
+    artfHipStreamDestroy(b->side);
     artfFree(b);
 }
 
@@ -131,8 +136,10 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     if (artfHipDeviceInfo(device, name, sizeof name, &b->computeUnits) || b->computeUnits < 1) return fail(b, "read the device");
     b->clockHz = artfHipWallClockHz(device);
     b->stream = artfHipStreamCreate();
-    bool ready = b->stream != NULL;
-    for (int i = 0; i < 2; ++i) ready = ready && (b->events[i] = artfHipEventCreate()) != NULL;
+    b->side = artfHipStreamCreate();
+    bool ready = b->stream != NULL && b->side != NULL;
+    for (int i = 0; i < 2; ++i)
+        ready = ready && (b->events[i] = artfHipEventCreate()) != NULL && (b->order[i] = artfHipEventCreateOrder()) != NULL;
     if (!ready) return fail(b, "make its stream");
 
     /* the shared tables, painted once on the device */
@@ -182,7 +189,7 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
 
     const size_t slots = 2 * ARTF_HIP_COUNTER_SLOTS, stamps = 1 + (size_t)count * ARTF_HIP_STAMP_SLOTS;
     const size_t queueBytes = (size_t)ARTF_HIP_QUEUES * ARTF_HIP_QUEUE_STRIDE * 4;
-    const size_t workBytes = queueBytes + slots * 8 + stamps * 8;
+    const size_t workBytes = 2 * queueBytes + slots * 8 + stamps * 8;
     if (ready) {
         b->deviceViews = artfHipMalloc((size_t)count * sizeof(ArtfHipView));
         b->firstTile = artfHipMalloc((size_t)(count + 1) * sizeof(uint32_t));
@@ -200,29 +207,45 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     a->views = b->deviceViews, a->firstTile = b->firstTile, a->viewCount = count, a->tileBegin = 0, a->tileEnd = at;
     b->tiles = at;
     a->precision = t.precision, a->map = b->map, a->rings = b->rings, a->fluids = b->fluids;
-    a->queue = (uint32_t*)w;
-    a->counters = (unsigned long long*)(w + queueBytes);
-    a->stamps = (unsigned long long*)(w + queueBytes + slots * 8);
+    b->queues[0] = (uint32_t*)w, b->queues[1] = (uint32_t*)(w + queueBytes);
+    a->queue = b->queues[0];
+    a->counters = (unsigned long long*)(w + 2 * queueBytes);
+    a->stamps = (unsigned long long*)(w + 2 * queueBytes + slots * 8);
     b->countersHost = artfMalloc(slots * 8);
     b->stampsHost = artfCalloc(stamps, 8);
     b->schedule = resolved(b, NULL);
     return b;
 }
 
-/* The graph of one draw: the reset, the trace over all views (in slices of launchTiles tiles, the
- * queues cleared between them), each view's resolve. */
+/* The graph of one draw: the reset, the trace over all views, each view's resolve. In slices of
+ * launchTiles tiles, the slices take turns on two streams, each with its own set of queues (cleared
+ * before each of its slices): a slice's last waves, as the persistent kernel drains, run beside the next
+ * slice's first, instead of the device idling between launches. */
 static bool capture(ArtfHoleBatch* b) {
     ArtfHipBatchArgs* a = &b->args;
     a->schedule.workgroups = b->schedule.workgroups, a->schedule.chunkTiles = b->schedule.chunkTiles;
     a->schedule.refill = b->schedule.refill, a->schedule.stepsPerCheck = b->schedule.stepsPerCheck;
     const uint32_t slice = b->schedule.launchTiles > 0 ? (uint32_t)b->schedule.launchTiles : b->tiles;
+    const bool split = b->tiles > slice;
+    void* streams[2] = {b->stream, b->side};
     if (artfHipGraphBegin(b->stream)) return false;
-    a->tileBegin = 0, a->tileEnd = slice < b->tiles ? slice : b->tiles;
-    int status = artfHipBatchReset(a, b->stream);
-    for (uint32_t begin = 0; !status && begin < b->tiles; begin += slice) {
-        a->tileBegin = begin, a->tileEnd = b->tiles - begin > slice ? begin + slice : b->tiles;
-        if (begin > 0) status = artfHipBatchResetQueues(a, b->stream);
-        if (!status) status = artfHipTraceViews(a, b->stream);
+    ArtfHipBatchArgs x = *a;
+    x.queue = b->queues[0], x.tileBegin = 0, x.tileEnd = slice < b->tiles ? slice : b->tiles;
+    int status = artfHipBatchReset(&x, b->stream); /* queues of the first set, counters, stamps */
+    if (!status && split) { /* the second stream joins once the reset is done */
+        status = artfHipEventRecord(b->order[0], b->stream);
+        if (!status) status = artfHipStreamWait(b->side, b->order[0]);
+    }
+    uint32_t k = 0;
+    for (uint32_t begin = 0; !status && begin < b->tiles; begin += slice, ++k) {
+        x.queue = b->queues[k & 1];
+        x.tileBegin = begin, x.tileEnd = b->tiles - begin > slice ? begin + slice : b->tiles;
+        if (k > 0) status = artfHipBatchResetQueues(&x, streams[k & 1]);
+        if (!status) status = artfHipTraceViews(&x, streams[k & 1]);
+    }
+    if (!status && split) { /* and rejoins before the resolves */
+        status = artfHipEventRecord(b->order[1], b->side);
+        if (!status) status = artfHipStreamWait(b->stream, b->order[1]);
     }
     for (int v = 0; !status && v < b->count; ++v)
         if (b->views[v].side > 1) status = artfHipResolve(&b->views[v].frame, b->views[v].rays, b->views[v].hdr, b->stream);

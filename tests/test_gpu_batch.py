@@ -123,6 +123,26 @@ def test_graph_replays(gpu, renderer):
     b.close()
 
 
+def test_graph_replays_in_slices(gpu, renderer):
+    """The batch in slices of 7 tiles, which take turns on two streams with a set of queues each: every
+    replay of its graph, with other device work between them, gives the renderer's images and the same
+    counts."""
+    tracing = dict(precision="fp32", disc="painted")
+    b = gpu.HoleBatch(SMALL, gpu.tracing(**tracing))
+    b.set_schedule(0, 2, 64, 8, 7)
+    counts = None
+    for replay in range(3):
+        b.draw()
+        s = b.stats()
+        if counts is None:
+            counts = (s.rays, s.steps, s.laneSlots)
+        assert (s.rays, s.steps, s.laneSlots) == counts, f"replay {replay}"
+        for k, view in enumerate(SMALL):
+            assert same_bits(b.read_hdr(k), rendered(gpu, renderer, view, **tracing)), f"view {k}, replay {replay}"
+        rendered(gpu, renderer, SMALL[replay % len(SMALL)], precision="fp32", disc="analytic")  # other work between replays
+    b.close()
+
+
 def test_tune_keeps_the_image(gpu, renderer):
     tracing = dict(precision="fp32", disc="painted")
     b = gpu.HoleBatch(SMALL, gpu.tracing(**tracing))
