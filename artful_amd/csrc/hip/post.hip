@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kPost) void toneKernel(Planes hdr, Planes bloom, ui
     const int y = (int)blockIdx.y;
     for (int k = 0, x = (int)blockIdx.x * kPost * kToneRun + (int)threadIdx.x; k < kToneRun && x < width; ++k, x += kPost)
         tonePixel(hdr, bloom, out, l0, width, exposure, spread, x, y);
-    /* the last workgroup's lane 0 starts at a pixel (x < width), so the last row is written once it ends */
+    /* the stage's end: when the last workgroup in launch order is done (its lane 0 always has a pixel) */
     if (stamps.end && blockIdx.x == gridDim.x - 1 && blockIdx.y == gridDim.y - 1 && threadIdx.x == 0)
         __hip_atomic_store(stamps.end, (unsigned long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
