@@ -33,6 +33,10 @@ __device__ inline void stampStart(unsigned long long* at, unsigned long long* al
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream);
 /* FP32 with fused multiply-adds, FP64, and mixed (trace_fast.hip). */
 hipError_t launchTraceFast(const ArtfHipTraceArgs& a, hipStream_t stream);
+/* FP64 (trace_fp64.hip), which launchTraceFast, launchViewsFast and viewsOccupancyFast hand FP64 to. */
+hipError_t launchTrace64(const ArtfHipTraceArgs& a, hipStream_t stream);
+hipError_t launchViews64(const ArtfHipBatchArgs& a, hipStream_t stream);
+int viewsOccupancy64();
 /* The persistent tracer of a batch of views: FP32 (trace_exact.hip), the others (trace_fast.hip); and
  * the workgroups of it a compute unit holds. */
 hipError_t launchViewsExact(const ArtfHipBatchArgs& a, hipStream_t stream);
