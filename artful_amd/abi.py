@@ -53,7 +53,7 @@ class HoleTracing(C.Structure):
 class HoleStats(C.Structure):
     _fields_ = [("traceMs", C.c_double), ("resolveMs", C.c_double), ("bloomMs", C.c_double), ("toneMs", C.c_double),
                 ("uploadMs", C.c_double), ("rays", C.c_uint64), ("steps", C.c_uint64), ("laneSlots", C.c_uint64),
-                ("bloomLevels", C.c_int)]
+                ("bloomLevels", C.c_int), ("frames", C.c_uint32), ("spanMs", C.c_double)]
 
 
 class FrameStats(C.Structure):
@@ -70,7 +70,7 @@ class Surface(C.Structure):
 class RendererOptions(C.Structure):
     _fields_ = [("backend", C.c_int), ("surface", Surface), ("width", C.c_int), ("height", C.c_int),
                 ("noVsync", C.c_bool), ("threads", C.c_int), ("simd", C.c_int), ("antialias", C.c_bool),
-                ("hdr", C.c_bool), ("compressTextures", C.c_bool), ("pipelineCache", C.c_char_p), ("device", C.c_int)]
+                ("hdr", C.c_bool), ("compressTextures", C.c_bool), ("pipelineCache", C.c_char_p)]
 
 
 class Overlay(C.Structure):
@@ -87,7 +87,7 @@ class HipTraceArgs(C.Structure):
     _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("map", C.c_void_p), ("fluid", C.c_void_p),
                 ("out", C.c_void_p * 3), ("counters", C.c_void_p), ("precision", C.c_int), ("maxSteps", C.c_int),
                 ("escapeRadius", C.c_float), ("disc", C.c_int), ("emissivity", C.c_float), ("jitter", C.c_uint32),
-                ("rowBegin", C.c_int), ("rowEnd", C.c_int), ("startStamp", C.c_void_p)]
+                ("rowBegin", C.c_int), ("rowEnd", C.c_int), ("startStamp", C.c_void_p), ("spanStamp", C.c_void_p)]
 
 
 class HipFrameArgs(C.Structure):

@@ -26,8 +26,8 @@ DRIVER = PKG / "artful-blackhole"
 OBJ = ROOT / "_build" / "artful_hip"
 ARCH = os.environ.get("ARTF_OFFLOAD_ARCH", "gfx950")
 
-C_SOURCES = ["core/base.c", "core/pool.c", "scene/disc.c", "scene/fluid.c", "scene/hole.c", "scene/scene.c",
-             "render/renderer.c", "render/hip_backend.c", "render/hole_adjoint.c", "render/hole_batch.c",
+C_SOURCES = ["core/base.c", "core/pool.c", "scene/disc.c", "scene/fluid.c", "scene/hole.c", "scene/frame64.c", "scene/scene.c",
+             "render/renderer.c", "render/hip_renderer.c", "render/hip_scene.c", "render/hole_adjoint.c", "render/hole_batch.c",
              "render/device_disc.c"]
 # (source, contraction, more flags). The exact, FMA and mixed tracers are scheduled for instruction-level
 # parallelism (LLVM's iterative ILP strategy), at the same waves a SIMD and with the same bits: exact 3%,

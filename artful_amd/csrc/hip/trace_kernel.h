@@ -63,7 +63,7 @@ __device__ __forceinline__ void countSteps(unsigned long long* counters, int ste
 
 template <int Precision>
 __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
-    stampStart(a.startStamp);
+    stampStart(a.startStamp, a.spanStamp);
     const ArtfHoleFrame& f = a.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;

@@ -1,6 +1,8 @@
 /* artful-hip — the choice of a backend and the calls every backend answers (Artful 0.4.0
- * src/render/renderer.c:4-78, same contracts). */
-#include "renderer_internal.h"
+ * src/render/renderer.c:4-78, same contracts). This library has one backend, HIP: AUTO and HIP take
+ * it, and with no HIP device both give NULL (logged); asking by name for the reference's others gives
+ * NULL too. */
+#include "render/rendererInternal.h"
 
 const char* artfBackendName(ArtfBackend b) {
     switch (b) {
@@ -12,7 +14,6 @@ const char* artfBackendName(ArtfBackend b) {
     }
 }
 
-/* NULL only when a backend asked for by name cannot run here; AUTO takes HIP. */
 ArtfRenderer* artfRendererNew(const ArtfRendererOptions* o) {
     ArtfRenderer* r = NULL;
     if (o->backend == ARTF_BACKEND_AUTO || o->backend == ARTF_BACKEND_HIP) r = artfHipNew(o);
@@ -28,15 +29,6 @@ ArtfRendererOptions artfRendererDefaults(int width, int height) {
     return o;
 }
 
-ArtfHoleTracing artfHoleTracingDefaults(void) {
-    ArtfHoleTracing t;
-    memset(&t, 0, sizeof t);
-    t.precision = ARTF_PRECISION_FP32;
-    t.disc = ARTF_DISC_PAINTED;
-    t.emissivity = 3.f;
-    return t;
-}
-
 void artfRendererFree(ArtfRenderer* r) {
     if (r) r->vt->destroy(r);
 }
@@ -50,10 +42,4 @@ uint32_t* artfRendererPixels(ArtfRenderer* r) { return r->vt->pixels(r); }
 void artfRendererStats(const ArtfRenderer* r, ArtfFrameStats* out) {
     memset(out, 0, sizeof *out);
     r->vt->stats(r, out);
-}
-bool artfRendererReadHdr(ArtfRenderer* r, float* red, float* green, float* blue) { return r->vt->readHdr(r, red, green, blue); }
-void artfRendererSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* tracing) { r->vt->setHoleTracing(r, tracing); }
-bool artfRendererHoleStats(const ArtfRenderer* r, ArtfHoleStats* out) {
-    memset(out, 0, sizeof *out);
-    return r->vt->holeStats(r, out);
 }

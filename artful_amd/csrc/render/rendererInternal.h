@@ -1,6 +1,9 @@
 /* artful-hip — a renderer: an object with a table of functions, as Artful 0.4.0's
- * src/render/rendererInternal.h:7-24 (ArtfRendererVtbl, struct ArtfRenderer). This library has one
- * backend, HIP; a build that also links the reference's software or Vulkan backend adds their tables. */
+ * src/render/rendererInternal.h:7-24 (ArtfRendererVtbl, struct ArtfRenderer), slot for slot and field
+ * for field, under the same file name: the HIP backend (hip_renderer.c) includes it as
+ * "render/rendererInternal.h", and so compiles against this library's copy or the reference's own.
+ * This library has one backend; a build of the reference's tree with this backend added declares
+ * artfHipNew beside artfVulkanNew (its rendererInternal.h:32-33). */
 #ifndef ARTFUL_RENDERER_INTERNAL_H
 #define ARTFUL_RENDERER_INTERNAL_H
 
@@ -15,22 +18,17 @@ typedef struct ArtfRendererVtbl {
     uint32_t* (*pixels)(ArtfRenderer* r);
     void (*stats)(const ArtfRenderer* r, ArtfFrameStats* out);
     void (*destroy)(ArtfRenderer* r);
-    /* beyond the reference */
-    bool (*readHdr)(ArtfRenderer* r, float* red, float* green, float* blue);
-    void (*setHoleTracing)(ArtfRenderer* r, const ArtfHoleTracing* tracing);
-    bool (*holeStats)(const ArtfRenderer* r, ArtfHoleStats* out);
 } ArtfRendererVtbl;
 
 struct ArtfRenderer {
     const ArtfRendererVtbl* vt;
     ArtfBackend backend;
+    /* the reference's occlusion trial (renderer.c:43-66); triangle scenes are not drawn here */
+    int trialFrame;
+    double trialMs[2];
 };
 
 /* NULL when no HIP device can run here (logged). */
 ArtfRenderer* artfHipNew(const ArtfRendererOptions* options);
-
-/* For tests: the number of device calls that succeed before the device is reported lost, when not
- * below 0 (as the reference's artfVulkanSubmitsLeft, rendererInternal.h:27-30). */
-extern int artfHipCallsLeft;
 
 #endif

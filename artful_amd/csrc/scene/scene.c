@@ -1,5 +1,5 @@
-/* artful-hip — a scene: a camera and, perhaps, a black hole (Artful 0.4.0 src/scene/scene.c:27-78, 941,
- * as far as the hole's path goes). */
+/* artful-hip — a scene: a camera, a background and, perhaps, a black hole (Artful 0.4.0
+ * src/scene/scene.c:27-78, 941-942, 1285, as far as the hole's path goes). */
 #include "scene_internal.h"
 
 #include "artful/device_disc.h"
@@ -17,6 +17,8 @@ ArtfScene* artfSceneNew(void) {
     s->camera.up = artfVec3(0, 1, 0);
     s->camera.position = artfVec3(0, 0, 1);
     s->hole.discDevice = -1;
+    s->background = artfVec3(.05f, .06f, .08f); /* the reference's scene.c:39-40 */
+    s->backgroundAlpha = 1.f;
     return s;
 }
 
@@ -30,6 +32,8 @@ void artfSceneFree(ArtfScene* s) {
 }
 
 void artfSceneSetCamera(ArtfScene* s, const ArtfCamera* camera) { s->camera = *camera; }
+void artfSceneSetBackground(ArtfScene* s, ArtfVec3 color) { s->background = color; }
+void artfSceneSetBackgroundAlpha(ArtfScene* s, float alpha) { s->backgroundAlpha = artfClampf(alpha, 0, 1); }
 
 bool artfSceneHoleDesc(const ArtfScene* s, ArtfBlackHoleDesc* out) {
     if (!s->hole.on) return false;

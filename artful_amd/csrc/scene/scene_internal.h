@@ -4,6 +4,7 @@
 
 #include "artful/hole.h"
 #include "artful/scene.h"
+#include "frame64.h"
 
 typedef struct ArtfFluid ArtfFluid;
 
@@ -28,6 +29,8 @@ struct ArtfScene {
     uint64_t id;
     ArtfCamera camera;
     ArtfHole hole;
+    ArtfVec3 background;   /* what shows without a hole (scene.c:39-40 defaults) */
+    float backgroundAlpha;
 };
 
 /* ---- disc.c: the painted gas and the per-ring tables (Artful src/scene/hole.c:10-197) ---- */
@@ -61,8 +64,6 @@ void artfHoleFrame64(const ArtfScene* s, int width, int height, ArtfHoleFrame64*
 ArtfBlackHoleDesc artfHoleClamped(const ArtfBlackHoleDesc* desc);
 void artfHoleFrameOf(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double fluidTime, int width, int height, int levelsCap, ArtfHoleFrame* out,
                      uint32_t* bloomPixels);
-void artfHoleFrame64Of(const ArtfBlackHoleDesc* d, const ArtfCamera* c, int width, int height, ArtfHoleFrame64* out);
-void artfHoleFrame64At(const ArtfBlackHoleDesc* d, const ArtfCamera* c, double spin, const double eye[3], int width, int height, ArtfHoleFrame64* out);
 /* The black-body colour table alone (ARTF_HOLE_COLORS x RGB), as the map holds it. */
 void artfDiscColors(float* colors);
 

@@ -43,6 +43,8 @@ def lib() -> C.CDLL:
         "artfFluidField": ([P], C.POINTER(C.c_float)), "artfFluidTime": ([P], D),
         "artfRendererDefaults": ([I, I], abi.RendererOptions),
         "artfRendererNew": ([C.POINTER(abi.RendererOptions)], P), "artfRendererFree": ([P], None),
+        "artfHipRendererNew": ([C.POINTER(abi.RendererOptions), I], P),
+        "artfSceneSetBackground": ([P, abi.Vec3], None), "artfSceneSetBackgroundAlpha": ([P, F], None),
         "artfRendererBackend": ([P], I), "artfRendererDescription": ([P], C.c_char_p),
         "artfRendererResize": ([P, I, I], None), "artfRendererSetOverlay": ([P, C.POINTER(abi.Overlay)], None),
         "artfRendererDraw": ([P, P], B), "artfRendererReadPixels": ([P, P], B), "artfRendererPixels": ([P], P),
@@ -149,6 +151,12 @@ class Scene:
     def set_camera(self, cam: abi.Camera) -> None:
         self._L.artfSceneSetCamera(self.ptr, C.byref(cam))
 
+    def set_background(self, color, alpha: float | None = None) -> None:
+        """artfSceneSetBackground (0..1 RGB), and artfSceneSetBackgroundAlpha when alpha is given."""
+        self._L.artfSceneSetBackground(self.ptr, abi.Vec3(*color))
+        if alpha is not None:
+            self._L.artfSceneSetBackgroundAlpha(self.ptr, alpha)
+
     def set_disc_on_device(self, device: int) -> None:
         """Build the disc's map and step its fluid on HIP device `device` (-1: on the host, the default)."""
         self._L.artfSceneSetDiscOnDevice(self.ptr, device)
@@ -232,10 +240,9 @@ class Renderer:
         self._L = lib()
         o = self._L.artfRendererDefaults(width, height)
         o.backend = abi.BACKEND_HIP
-        o.device = device
-        self.ptr = self._L.artfRendererNew(C.byref(o))
+        self.ptr = self._L.artfHipRendererNew(C.byref(o), device)
         if not self.ptr:
-            raise RuntimeError("artfRendererNew: no HIP device can run the renderer here")
+            raise RuntimeError(f"artfHipRendererNew: HIP device {device} cannot run the renderer here")
         self.width, self.height = width, height
 
     def description(self) -> str:

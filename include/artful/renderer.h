@@ -1,17 +1,22 @@
 /* artful-hip — drawing a scene's black hole on an MI355X (gfx950).
  *
- * The renderer API of Artful 0.4.0 (include/artful/renderer.h:24-133) with one more backend,
- * ARTF_BACKEND_HIP, appended after the reference's values so they keep their numbers. The calls
- * keep their contracts: artfRendererNew returns NULL only when a backend asked for by name cannot
- * run; artfRendererDraw returns false when the device is lost (free the renderer, keep the scene);
- * artfRendererReadPixels writes width*height BGRA8, alpha 255; artfRendererPixels is NULL for a GPU.
+ * The renderer API of Artful 0.4.0 (include/artful/renderer.h:24-133): the same names, enum values,
+ * struct layouts and contracts, byte for byte (sizes and offsets are checked against the reference's
+ * own build), with one more backend, ARTF_BACKEND_HIP, appended after the reference's values so they
+ * keep their numbers. artfRendererDraw returns false when the device is lost (free the renderer, keep
+ * the scene); artfRendererReadPixels writes width*height premultiplied BGRA8; artfRendererPixels is
+ * NULL for a GPU.
  *
- * Beyond the reference: artfRendererReadHdr (the traced light before bloom, which the reference keeps
- * private) and artfRendererSetHoleTracing (precision, disc model and step law). */
+ * One deviation: the reference's AUTO falls back to its software backend where no GPU runs
+ * (renderer.h:79, renderer.c:24), so it never gives NULL. This library has the HIP backend only, so
+ * here AUTO gives NULL (logged) where no HIP device runs. Built into the reference's tree (its src/hip)
+ * the backend leaves the reference's dispatch, and that fallback, as they are.
+ *
+ * What the HIP backend adds (HDR readback, precision and ray law, per-stage GPU times, the choice of
+ * a device) is declared in artful/hip_renderer.h, included at the end. */
 #ifndef ARTFUL_RENDERER_H
 #define ARTFUL_RENDERER_H
 
-#include "artful/hole.h"
 #include "artful/scene.h"
 
 #ifdef __cplusplus
@@ -19,7 +24,7 @@ extern "C" {
 #endif
 
 typedef enum ArtfBackend {
-    ARTF_BACKEND_AUTO,
+    ARTF_BACKEND_AUTO,     /* HIP here (the reference: Vulkan where it works, then software) */
     ARTF_BACKEND_VULKAN,   /* not in this library */
     ARTF_BACKEND_SOFTWARE, /* not in this library */
     ARTF_BACKEND_METAL,    /* not in this library */
@@ -28,13 +33,26 @@ typedef enum ArtfBackend {
 
 const char* artfBackendName(ArtfBackend b);
 
-typedef enum ArtfSurfaceKind { ARTF_SURFACE_NONE } ArtfSurfaceKind;
-typedef struct ArtfSurface { ArtfSurfaceKind kind; void* a; void* b; } ArtfSurface;
+typedef enum ArtfSurfaceKind {
+    ARTF_SURFACE_NONE,    /* frames stay in memory: artfRendererReadPixels (the only kind the HIP backend draws to) */
+    ARTF_SURFACE_WIN32,   /* a: the HWND */
+    ARTF_SURFACE_XLIB,    /* a: the Display*, b: the Window */
+    ARTF_SURFACE_WAYLAND, /* a: the wl_display*, b: the wl_surface* */
+    ARTF_SURFACE_ANDROID, /* a: the ANativeWindow* */
+    ARTF_SURFACE_METAL    /* a: the CAMetalLayer* */
+} ArtfSurfaceKind;
+
+typedef struct ArtfSurface {
+    ArtfSurfaceKind kind;
+    void* a;
+    void* b;
+} ArtfSurface;
+
 typedef enum ArtfSimd { ARTF_SIMD_AUTO, ARTF_SIMD_PORTABLE, ARTF_SIMD_128, ARTF_SIMD_256 } ArtfSimd;
 
 typedef struct ArtfRendererOptions {
     ArtfBackend backend; /* AUTO: HIP */
-    ArtfSurface surface; /* frames stay in memory */
+    ArtfSurface surface; /* the HIP backend draws into memory whatever is given */
     int width, height;
     bool noVsync;
     int threads;         /* host threads for the disc's tables and fluid: 0 as many as allowed, at most 16 */
@@ -43,16 +61,19 @@ typedef struct ArtfRendererOptions {
     bool hdr;
     bool compressTextures;
     const char* pipelineCache;
-    int device;          /* HIP: which GPU (beyond the reference) */
 } ArtfRendererOptions;
 
+/* Fields set by name on this stay right when the struct grows. */
 ArtfRendererOptions artfRendererDefaults(int width, int height);
 
 typedef struct ArtfRenderer ArtfRenderer;
 
+/* NULL only when a backend was asked for by name and cannot run here (and, in this library, for AUTO
+ * with no HIP device: see above). */
 ArtfRenderer* artfRendererNew(const ArtfRendererOptions* options);
 void artfRendererFree(ArtfRenderer* r);
 ArtfBackend artfRendererBackend(const ArtfRenderer* r);
+/* What draws: "HIP, AMD Instinct MI355X, 256 compute units". Borrowed. */
 const char* artfRendererDescription(const ArtfRenderer* r);
 void artfRendererResize(ArtfRenderer* r, int width, int height);
 
@@ -60,7 +81,7 @@ typedef struct ArtfRect { int x, y, w, h; } ArtfRect;
 typedef struct ArtfOverlay {
     const uint32_t* pixels;
     int width, height;
-    int stride;
+    int stride; /* in pixels; 0: width */
     bool partial;
     const ArtfRect* damage;
     size_t damageCount;
@@ -90,38 +111,10 @@ typedef struct ArtfFrameStats {
 
 void artfRendererStats(const ArtfRenderer* r, ArtfFrameStats* out);
 
-/* ---- beyond the reference ---- */
-
-/* The last frame's light before bloom: three planes (R, G, B) of width*height floats. */
-bool artfRendererReadHdr(ArtfRenderer* r, float* red, float* green, float* blue);
-
-typedef struct ArtfHoleTracing {
-    int precision;      /* ArtfPrecision */
-    int disc;           /* ArtfDiscModel */
-    float emissivity;   /* q of the analytic disc */
-    int maxSteps;       /* 0: 192, the reference's */
-    float escapeRadius; /* 0: a ray is away where 1/r crosses 0, the reference's rule */
-    uint32_t jitter;    /* 0: rays on the reference's stratified grid; else a seed for jittered rays */
-    int bloomLevels;    /* 0: the reference's rule (while both sides exceed 3, up to 8) */
-    int reserved;
-} ArtfHoleTracing;
-
-ArtfHoleTracing artfHoleTracingDefaults(void);
-void artfRendererSetHoleTracing(ArtfRenderer* r, const ArtfHoleTracing* tracing);
-
-/* The last frame's hole, stage by stage on the GPU's clock, and the tracer's work. */
-typedef struct ArtfHoleStats {
-    double traceMs, resolveMs, bloomMs, toneMs, uploadMs;
-    uint64_t rays;      /* width * height * samples^2 */
-    uint64_t steps;     /* Nystrom steps, summed over rays */
-    uint64_t laneSlots; /* lanes x steps the waves spent stepping: steps / laneSlots is the active-lane fraction */
-    int bloomLevels;
-} ArtfHoleStats;
-
-bool artfRendererHoleStats(const ArtfRenderer* r, ArtfHoleStats* out);
-
 #ifdef __cplusplus
 }
 #endif
+
+#include "artful/hip_renderer.h"
 
 #endif

@@ -1,7 +1,7 @@
-/* artful-hip — the scene, as far as the black-hole path needs it: a camera and a hole.
+/* artful-hip — the scene, as far as the black-hole path needs it: a camera, a background and a hole.
  *
  * Same names, layouts and meanings as Artful 0.4.0's include/artful/scene.h:312-322 (ArtfCamera,
- * artfSceneSetCamera) and :465-500 (ArtfBlackHoleDesc, artfBlackHoleDefaults,
+ * artfSceneSetCamera), :343-347 (artfSceneSetBackground, artfSceneSetBackgroundAlpha) and :465-500 (ArtfBlackHoleDesc, artfBlackHoleDefaults,
  * artfBlackHoleOrbitSpeed, artfSceneSetBlackHole); artfSceneNew/artfSceneFree as in
  * src/scene/scene.c:27-52. Triangle scenes (meshes, materials, lights, glTF) are not part of this
  * library. A scene is used from one thread, and does not change while a renderer draws it. */
@@ -29,6 +29,13 @@ typedef struct ArtfCamera {
 } ArtfCamera;
 
 void artfSceneSetCamera(ArtfScene* s, const ArtfCamera* camera);
+
+/* What shows where nothing is drawn, 0..1 for red, green, blue (scene.h:343-344): here, a frame with no
+ * hole. (.05, .06, .08) to start with. */
+void artfSceneSetBackground(ArtfScene* s, ArtfVec3 color);
+/* How much of the background there is, 0..1 (1 to start with): a frame with no hole has this alpha and
+ * its colour premultiplied by it; a black hole fills the frame all the same (scene.h:345-347). */
+void artfSceneSetBackgroundAlpha(ArtfScene* s, float alpha);
 
 /* A Kerr black hole with a glowing disc, lengths in masses (scene.h:465-490). 92 bytes. */
 typedef struct ArtfBlackHoleDesc {

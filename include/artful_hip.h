@@ -90,6 +90,9 @@ typedef struct ArtfHipTraceArgs {
     /* NULL, or a word (device or artfHipHostMalloc memory) the launch's first workgroup sets to the GPU's
      * clock (artfHipWallClockHz) as it starts */
     unsigned long long* startStamp;
+    /* NULL, or a second word set with startStamp, at the same instant (a caller that queues several
+     * frames gives it to the first, to time the run of them) */
+    unsigned long long* spanStamp;
 } ArtfHipTraceArgs;
 
 int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
@@ -248,7 +251,7 @@ int artfHipToneMap(const ArtfHoleFrame* frame, float* const hdr[3], float* const
 #define ARTF_HIP_REPORT_WORDS 7
 typedef struct ArtfHipFrameArgs {
     /* out[]: the rays' planes (side > 1) or the HDR planes (side 1); whole frames (rowBegin 0, rowEnd 0
-     * or the height); startStamp: set by artfHipDrawFrame. counters: required with a report, and zero
+     * or the height); startStamp: set by artfHipDrawFrame (spanStamp is kept as given). counters: required with a report, and zero
      * before the first frame (each frame leaves them zero). */
     ArtfHipTraceArgs trace;
     float* hdr[3];   /* device, width * height each */

@@ -1,6 +1,7 @@
 """Builds the oracle (oracle/liboracle.so) from oracle/lens.c with the reference's floating-point
 rules (-ffp-contract=off -fno-math-errno -fno-trapping-math, Artful CMakeLists.txt:54-58); and,
-where the reference tree is present, oracle/_ref/libartful_ref.so from it (oracle/ref/Makefile).
+where the reference tree is present, oracle/_ref/libartful_ref.so from it (oracle/ref/Makefile), and
+oracle/_ref/libartful_routeb.so (the reference's tree with the HIP backend added, route B).
 Test infrastructure only."""
 from __future__ import annotations
 
@@ -30,6 +31,9 @@ def build_reference() -> Path | None:
         return None
     jobs = str(min(8, os.cpu_count() or 1))
     subprocess.run(["make", "-s", "-C", str(HERE / "ref"), f"-j{jobs}", f"REF={REF}"], check=True)
+    # route B: the reference's tree with this repository's HIP backend added (needs the kernels' objects,
+    # which artful_amd/build.py leaves in _build/artful_hip)
+    subprocess.run(["make", "-s", "-C", str(HERE / "ref"), f"-j{jobs}", f"REF={REF}", "routeb"], check=True)
     return HERE / "_ref" / "libartful_ref.so"
 
 

@@ -131,6 +131,8 @@ def reference() -> C.CDLL:
             "refRendererNew": ([I, I, I, I], P), "refRendererDraw": ([P, P], I), "refRendererRead": ([P, P], None),
             "refRendererStats": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], None), "refRendererFree": ([P], None),
             "refRendererDescription": ([P], C.c_char_p),
+            "refLayoutCount": ([], I), "refLayoutType": ([I], C.c_char_p), "refLayoutField": ([I], C.c_char_p),
+            "refLayoutAt": ([I], C.c_long), "refEnumValue": ([I], I),
         }
         for name, (args, ret) in sig.items():
             fn = getattr(L, name)

@@ -116,3 +116,47 @@ void refRendererStats(const void* r, double* lensMs, double* rasterMs, double* c
 }
 void refRendererFree(void* r) { artfRendererFree((ArtfRenderer*)r); }
 const char* refRendererDescription(const void* r) { return artfRendererDescription((const ArtfRenderer*)r); }
+
+/* The layouts a drop-in must match: sizeof of each public struct (field "") and the offset of each of
+ * its fields, as the reference's headers lay them out (include/artful/renderer.h, scene.h, core.h;
+ * src/scene/sceneInternal.h for the frame's uniform block). */
+#include <stddef.h>
+#define SIZE(T) {#T, "", sizeof(T)}
+#define AT(T, f) {#T, #f, offsetof(T, f)}
+static const struct { const char* type; const char* field; size_t at; } layouts[] = {
+    SIZE(ArtfRendererOptions), AT(ArtfRendererOptions, backend), AT(ArtfRendererOptions, surface), AT(ArtfRendererOptions, width),
+    AT(ArtfRendererOptions, height), AT(ArtfRendererOptions, noVsync), AT(ArtfRendererOptions, threads), AT(ArtfRendererOptions, simd),
+    AT(ArtfRendererOptions, antialias), AT(ArtfRendererOptions, hdr), AT(ArtfRendererOptions, compressTextures),
+    AT(ArtfRendererOptions, pipelineCache),
+    SIZE(ArtfSurface), AT(ArtfSurface, kind), AT(ArtfSurface, a), AT(ArtfSurface, b),
+    SIZE(ArtfFrameStats), AT(ArtfFrameStats, cpuMs), AT(ArtfFrameStats, cullMs), AT(ArtfFrameStats, geometryMs), AT(ArtfFrameStats, rasterMs),
+    AT(ArtfFrameStats, lensMs), AT(ArtfFrameStats, instancesDrawn), AT(ArtfFrameStats, trianglesBinned), AT(ArtfFrameStats, tilesSkipped),
+    AT(ArtfFrameStats, instancesHidden), AT(ArtfFrameStats, traceSamples), AT(ArtfFrameStats, gpuMs), AT(ArtfFrameStats, shadowMapsDrawn),
+    SIZE(ArtfOverlay), AT(ArtfOverlay, pixels), AT(ArtfOverlay, width), AT(ArtfOverlay, height), AT(ArtfOverlay, stride),
+    AT(ArtfOverlay, partial), AT(ArtfOverlay, damage), AT(ArtfOverlay, damageCount),
+    SIZE(ArtfRect), AT(ArtfRect, x), AT(ArtfRect, y), AT(ArtfRect, w), AT(ArtfRect, h),
+    SIZE(ArtfVec3), SIZE(ArtfCamera), AT(ArtfCamera, position), AT(ArtfCamera, target), AT(ArtfCamera, up), AT(ArtfCamera, fovY),
+    AT(ArtfCamera, near_),
+    SIZE(ArtfBlackHoleDesc), AT(ArtfBlackHoleDesc, position), AT(ArtfBlackHoleDesc, axis), AT(ArtfBlackHoleDesc, mass),
+    AT(ArtfBlackHoleDesc, spin), AT(ArtfBlackHoleDesc, discInner), AT(ArtfBlackHoleDesc, discOuter), AT(ArtfBlackHoleDesc, discHeight),
+    AT(ArtfBlackHoleDesc, discThickness), AT(ArtfBlackHoleDesc, discGain), AT(ArtfBlackHoleDesc, doppler), AT(ArtfBlackHoleDesc, stars),
+    AT(ArtfBlackHoleDesc, bloom), AT(ArtfBlackHoleDesc, exposure), AT(ArtfBlackHoleDesc, time), AT(ArtfBlackHoleDesc, shutter),
+    AT(ArtfBlackHoleDesc, velocity), AT(ArtfBlackHoleDesc, samples),
+    SIZE(ArtfHoleFrame), AT(ArtfHoleFrame, er), AT(ArtfHoleFrame, etheta), AT(ArtfHoleFrame, ephi), AT(ArtfHoleFrame, right),
+    AT(ArtfHoleFrame, up), AT(ArtfHoleFrame, forward), AT(ArtfHoleFrame, metric), AT(ArtfHoleFrame, place), AT(ArtfHoleFrame, spin),
+    AT(ArtfHoleFrame, flow), AT(ArtfHoleFrame, boost), AT(ArtfHoleFrame, disc), AT(ArtfHoleFrame, look), AT(ArtfHoleFrame, image),
+    AT(ArtfHoleFrame, size), AT(ArtfHoleFrame, bloom),
+};
+int refLayoutCount(void) { return (int)(sizeof layouts / sizeof layouts[0]); }
+const char* refLayoutType(int i) { return layouts[i].type; }
+const char* refLayoutField(int i) { return layouts[i].field; }
+long refLayoutAt(int i) { return (long)layouts[i].at; }
+/* enum values a drop-in must keep */
+int refEnumValue(int which) {
+    switch (which) {
+    case 0: return ARTF_BACKEND_METAL;
+    case 1: return ARTF_SURFACE_METAL;
+    case 2: return ARTF_SIMD_256;
+    default: return -1;
+    }
+}

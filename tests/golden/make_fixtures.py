@@ -7,7 +7,8 @@ AVX2 and portable kernels (lens.inc via kernels*.c), whole software frames (BGRA
 orbit and orbit speeds. Large outputs are stored as SHA-256 digests of their bytes; a few small HDR
 frames are stored whole.
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py              (everything)
+    python tests/golden/make_fixtures.py --layouts    (only the struct layouts and enum values)
 """
 from __future__ import annotations
 
@@ -33,7 +34,23 @@ def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
 
 
+def layouts(L) -> dict:
+    """sizeof and offsetof of the public structs, and the last value of each public enum, as the
+    reference's headers give them (oracle/ref/ref_driver.c's table)."""
+    out: dict = {}
+    for i in range(L.refLayoutCount()):
+        out.setdefault(L.refLayoutType(i).decode(), {})[L.refLayoutField(i).decode() or "sizeof"] = int(L.refLayoutAt(i))
+    out["enums"] = {"ARTF_BACKEND_METAL": L.refEnumValue(0), "ARTF_SURFACE_METAL": L.refEnumValue(1), "ARTF_SIMD_256": L.refEnumValue(2)}
+    return out
+
+
 def main() -> None:
+    if "--layouts" in sys.argv:  # only the layouts, into the fixtures already written
+        fx = json.loads((HERE / "fixtures.json").read_text())
+        fx["layouts"] = layouts(O.reference())
+        (HERE / "fixtures.json").write_text(json.dumps(fx, indent=1, sort_keys=True))
+        print("wrote the layouts into", HERE / "fixtures.json")
+        return
     if not O.reference_available():
         raise SystemExit("oracle/_ref/libartful_ref.so is not built (python oracle/build.py, with the reference tree present)")
     L = O.reference()
@@ -86,6 +103,7 @@ def main() -> None:
             hs.append([t, digest(np.ctypeslib.as_array(L.refFluidField(f), shape=(abi.FLUID_WIDTH * abi.FLUID_HEIGHT,)).copy())])
         L.refFluidFree(f)
         out["fluid_alone"][f"{inner!r},{outer!r},{spin!r}"] = hs
+    out["layouts"] = layouts(L)
     (HERE / "fixtures.json").write_text(json.dumps(out, indent=1, sort_keys=True))
     np.savez_compressed(HERE / "hdr_small.npz", **arrays)
     print("wrote", HERE / "fixtures.json", HERE / "hdr_small.npz")

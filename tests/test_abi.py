@@ -4,10 +4,13 @@ from __future__ import annotations
 
 import ctypes as C
 import subprocess
+from pathlib import Path
 
 import pytest
 
 from artful_amd import abi
+
+ROOT = Path(__file__).resolve().parent.parent
 
 
 def exported(path):
@@ -27,9 +30,75 @@ def test_struct_sizes():
     assert C.sizeof(abi.BlackHoleDesc) == 92
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
-    assert C.sizeof(abi.HipTraceArgs) == 808 and C.sizeof(abi.HipFrameArgs) == 896
+    assert C.sizeof(abi.HipTraceArgs) == 816 and C.sizeof(abi.HipFrameArgs) == 904
     assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 96 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
     assert abi.MAP_FLOATS == 2887424
+
+
+LAYOUT_TYPES = {"ArtfRendererOptions": abi.RendererOptions, "ArtfSurface": abi.Surface, "ArtfFrameStats": abi.FrameStats,
+                "ArtfOverlay": abi.Overlay, "ArtfVec3": abi.Vec3, "ArtfCamera": abi.Camera, "ArtfBlackHoleDesc": abi.BlackHoleDesc,
+                "ArtfHoleFrame": abi.HoleFrame}
+
+
+def test_public_layouts_are_the_reference_s(golden, tmp_path):
+    """Every public struct a program compiled against the reference hands this library (options, stats,
+    overlay, camera, hole description; and the frame's uniform block) has the reference's size and
+    field offsets, and the enums their values: sizeof/offsetof of the reference's own headers
+    (oracle/ref/ref_driver.c) against this library's headers, compiled here, and its ctypes mirrors."""
+    want = golden["layouts"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "artful/renderer.h"', '#include "artful/hole.h"', 'int main(void) {']
+    for ty, fields in want.items():
+        if ty == "enums":
+            continue
+        for field in fields:
+            expr = f"sizeof({ty})" if field == "sizeof" else f"offsetof({ty}, {field})"
+            lines.append(f'    printf("{ty} {field} %zu\\n", {expr});')
+    for name in want["enums"]:
+        lines.append(f'    printf("enums {name} %d\\n", (int){name});')
+    lines.append("    return 0;\n}")
+    src = tmp_path / "layouts.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layouts"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got: dict = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        ty, field, value = line.split()
+        got.setdefault(ty, {})[field] = int(value)
+    assert got == want
+    for ty, cls in LAYOUT_TYPES.items():
+        assert C.sizeof(cls) == want[ty]["sizeof"], ty
+        for field, at in want[ty].items():
+            if field != "sizeof":
+                assert getattr(cls, field).offset == at, (ty, field)
+
+
+REFERENCE = Path("/root/reference")
+
+
+@pytest.mark.skipif(not (REFERENCE / "src" / "render" / "rendererInternal.h").exists(), reason="the reference tree is not here")
+def test_route_b_compiles_against_the_reference(tmp_path):
+    """The HIP backend (hip_renderer.c) and the route-B scene port compile against the reference's own
+    headers, the reference's include/ and src/ ahead of this repository's, with every warning on: the
+    backend reads the reference's scene and fills the reference's vtable as they are."""
+    csrc = ROOT / "artful_amd" / "csrc"
+    for f in ("render/hip_renderer.c", "render/route_b/hip_scene_reference.c", "scene/frame64.c"):
+        r = subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter",
+                            "-Wno-sign-compare", f"-I{REFERENCE / 'include'}", f"-I{REFERENCE / 'src'}", f"-I{ROOT / 'include'}",
+                            f"-I{csrc}", "-M", str(csrc / f)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        deps = r.stdout
+        assert str(REFERENCE / "include" / "artful" / "scene.h") in deps and str(ROOT / "include" / "artful" / "scene.h") not in deps
+        if f == "render/hip_renderer.c":  # the reference's vtable and renderer API
+            assert str(REFERENCE / "src" / "render" / "rendererInternal.h") in deps
+            assert str(REFERENCE / "include" / "artful" / "renderer.h") in deps
+            assert str(csrc / "render" / "rendererInternal.h") not in deps
+        if "route_b" in f:  # the reference's scene as the Vulkan backend reads it
+            assert str(REFERENCE / "src" / "scene" / "sceneInternal.h") in deps
+            assert "scene_internal.h" not in deps
+        r = subprocess.run(["gcc", "-std=c11", "-fsyntax-only", "-Wall", "-Wextra", "-Werror", "-Wno-unused-parameter",
+                            "-Wno-sign-compare", f"-I{REFERENCE / 'include'}", f"-I{REFERENCE / 'src'}", f"-I{ROOT / 'include'}",
+                            f"-I{csrc}", str(csrc / f)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
 
 
 def test_backend_names(native):

@@ -217,17 +217,35 @@ def test_frame_beyond_the_grid_is_refused(gpu, scene_of):
         r.close()
 
 
+def over(s, d):
+    """The reference's overlay rule (kernels.inc:1264-1272, 1748-1756): every channel, alpha too,
+    s + d (255 - sa) / 255, the division rounded as (t + (t >> 8)) >> 8 of t = d (255 - sa) + 128."""
+    rest, out = 255 - (s >> 24), 0
+    for c in range(0, 32, 8):
+        t = ((d >> c) & 255) * rest + 128
+        out |= ((t + (t >> 8)) >> 8) << c
+    return (s + out) & 0xFFFFFFFF
+
+
 def test_no_hole_and_overlay(renderer, native):
     s = native.Scene()
     renderer.resize(8, 4)
     assert renderer.draw(s)
     px = renderer.read_pixels()
-    assert (px == 0xFF0D0F14).all()
+    assert (px == 0xFF0D0F14).all()  # the reference's default background (.05, .06, .08), packed as software.c:597-598
     overlay = np.zeros((4, 8), np.uint32)
     overlay[1, 2] = 0xFFFF0000
     overlay[2, 3] = 0x80400000
+    overlay[3, 5] = 0x00102030  # premultiplied, alpha 0: adds its colour
     renderer.set_overlay(overlay)
     px = renderer.read_pixels()
     renderer.set_overlay(None)
     assert px[1, 2] == 0xFFFF0000 and px[0, 0] == 0xFF0D0F14
-    assert (px[2, 3] >> 24) & 255 == 255 and (px[2, 3] >> 16) & 255 == 0x40 + (0x0D * 127 + 127) // 255
+    assert px[2, 3] == over(0x80400000, 0xFF0D0F14) and (px[2, 3] >> 16) & 255 == 0x40 + 6
+    assert px[3, 5] == 0xFF1D2F44
+    s.set_background((0, 0, 1))
+    assert renderer.draw(s)
+    assert (renderer.read_pixels() == 0xFF0000FF).all()
+    s.set_background((1, .5, 0), alpha=.5)  # premultiplied by its alpha: (.5, .25, 0) at 128
+    assert renderer.draw(s)
+    assert (renderer.read_pixels() == (128 << 24 | 128 << 16 | 64 << 8)).all()
