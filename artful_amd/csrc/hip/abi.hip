@@ -194,7 +194,7 @@ int artfHipDrawFrame(const ArtfHipFrameArgs* a, void* stream) {
     const ArtfHoleFrame* f = &a->trace.frame;
     if (a->trace.rowBegin != 0 || (a->trace.rowEnd != 0 && a->trace.rowEnd < (int)f->size[1])) return (int)hipErrorInvalidValue; /* whole frames */
     for (int c = 0; c < 3; ++c)
-        if (!a->hdr[c] || !a->bloom[c] || (f->size[2] == 1 && a->trace.out[c] != a->hdr[c])) return (int)hipErrorInvalidValue;
+        if (!a->hdr[c] || !a->bloom[c] || a->trace.out[c] != a->hdr[c]) return (int)hipErrorInvalidValue;
     if (a->postStream && (!a->handoff || a->postStream == stream)) return (int)hipErrorInvalidValue;
     return (int)launchFrame(*a, fresh(stream));
 }

@@ -1,4 +1,5 @@
-/* artful-hip — after the trace: each pixel's rays averaged, the bloom pyramid, and the tone map.
+/* artful-hip — after the trace: the bloom pyramid and the tone map (and, for a batch, each pixel's rays
+ * averaged, which the tile tracer does in its own lanes).
  * All three are HBM-bound streams over float planes, one texel per lane with consecutive lanes on
  * consecutive texels of a row (coalesced). Built with -ffp-contract=off: the arithmetic is the
  * reference's, operation for operation (src/software/software.c:320-386; lens.inc:364-380, 387-440),
@@ -88,55 +89,6 @@ __global__ __launch_bounds__(kPost) void bloomFirstKernel(Planes hdr, Planes blo
         const float* b = hdr.p[c] + (size_t)y1 * width;
 #define BRIGHT(v) minRef(maxRef((v) - ARTF_BLOOM_THRESHOLD, 0.f), ARTF_BLOOM_CEILING)
         bloom.p[c][to.at + (size_t)y * to.w + x] = (BRIGHT(a[x0]) + BRIGHT(a[x1]) + BRIGHT(b[x0]) + BRIGHT(b[x1])) * .25f;
-#undef BRIGHT
-    }
-}
-
-/* A frame's resolve and level 0 in one pass: each thread resolves the 2 x 2 pixels under its level-0
- * texel (resolveKernel's sums, each in ray order), then stores their HDR (each pixel by the one texel
- * that holds it) and their level-0 mean, bloomFirstKernel's, from registers. Ray by ray, the twelve
- * values it needs are loaded together (two pixels of a row as one 8-byte load where the row allows), and
- * all loads come before the first store: a load issued after a store would wait for it. */
-__global__ __launch_bounds__(kPost) void resolveFirstKernel(Planes rays, Planes hdr, Planes bloom, Level to, int width, int height, int count,
-                                                          float share, unsigned long long* stamp, Settle z) {
-    stampStart(stamp);
-    settleCounters(z);
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= (int)to.w) return;
-    const int lastX = width - 1, lastY = height - 1, pixels = width * height;
-    const int ys[2] = {min(y * 2, lastY), min(y * 2 + 1, lastY)}, xs[2] = {min(x * 2, lastX), min(x * 2 + 1, lastX)};
-    const bool pair = (width & 1) == 0 && x * 2 + 1 <= lastX; /* both pixels there, 8-byte aligned */
-    float v[3][2][2] = {};
-    for (int k = 0; k < count; ++k) {
-        float got[3][2][2];
-        for (int c = 0; c < 3; ++c)
-            for (int dy = 0; dy < 2; ++dy) {
-                const float* row = rays.p[c] + (size_t)k * pixels + ys[dy] * width;
-                if (pair) {
-                    const float2 two = *(const float2*)(row + xs[0]);
-                    got[c][dy][0] = two.x, got[c][dy][1] = two.y;
-                } else {
-                    got[c][dy][0] = row[xs[0]], got[c][dy][1] = row[xs[1]];
-                }
-            }
-        for (int c = 0; c < 3; ++c)
-            for (int dy = 0; dy < 2; ++dy)
-                for (int dx = 0; dx < 2; ++dx) v[c][dy][dx] = v[c][dy][dx] + got[c][dy][dx];
-    }
-    for (int c = 0; c < 3; ++c)
-        for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx) v[c][dy][dx] = v[c][dy][dx] * share;
-    for (int c = 0; c < 3; ++c) {
-        for (int dy = 0; dy < 2; ++dy) {
-            float* row = hdr.p[c] + ys[dy] * width;
-            if (y * 2 + dy > lastY) continue;
-            if (pair) *(float2*)(row + xs[0]) = make_float2(v[c][dy][0], v[c][dy][1]);
-            else
-                for (int dx = 0; dx < 2; ++dx)
-                    if (x * 2 + dx <= lastX) row[xs[dx]] = v[c][dy][dx];
-        }
-#define BRIGHT(v) minRef(maxRef((v) - ARTF_BLOOM_THRESHOLD, 0.f), ARTF_BLOOM_CEILING)
-        bloom.p[c][to.at + (size_t)y * to.w + x] = (BRIGHT(v[c][0][0]) + BRIGHT(v[c][0][1]) + BRIGHT(v[c][1][0]) + BRIGHT(v[c][1][1])) * .25f;
 #undef BRIGHT
     }
 }
@@ -373,8 +325,7 @@ hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
     t.startStamp = r ? r + 2 : nullptr;
     const hipError_t traced = t.precision == ARTF_PRECISION_FP32 ? launchTraceExact(t, stream) : launchTraceFast(t, stream);
     if (traced != hipSuccess) return traced;
-    const int side = (int)f.size[2];
-    Planes rays = {{t.out[0], t.out[1], t.out[2]}}, h = {{a.hdr[0], a.hdr[1], a.hdr[2]}}, b = {{a.bloom[0], a.bloom[1], a.bloom[2]}};
+    Planes h = {{a.hdr[0], a.hdr[1], a.hdr[2]}}, b = {{a.bloom[0], a.bloom[1], a.bloom[2]}};
     const Level l0 = levelOf(f, 0);
     const Settle z = {t.counters, r};
     /* the rest on the second stream, once the trace is done: the first can take the next frame's trace */
@@ -385,11 +336,7 @@ hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
         if ((e = hipEventRecord((hipEvent_t)a.handoff, stream)) != hipSuccess || (e = hipStreamWaitEvent(post, (hipEvent_t)a.handoff, 0)) != hipSuccess)
             return e;
     }
-    if (side > 1)
-        hipLaunchKernelGGL(resolveFirstKernel, rowsOf(l0), dim3(kPost), 0, post, rays, h, b, l0, (int)f.size[0], (int)f.size[1], side * side, f.image[3],
-                           r ? r + 3 : nullptr, z);
-    else
-        hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
+    hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
     const bool rest = bloomRest(f, b, post, r ? r + 4 : nullptr);
     const ToneStamps ts = r ? ToneStamps{r + 5, rest ? nullptr : r + 4, r + 6} : ToneStamps{};
     hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(kPost), 0, post, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), ts);

@@ -1,8 +1,11 @@
-/* artful-hip — the tracing kernel: a workgroup of one wave traces an 8 x 8 tile of pixels (neighbouring
- * rays, which take similar paths, share the wave) for one of the pixel's side^2 rays (grid z). One wave
- * a workgroup because rays end at their own step: a wave's slot on its SIMD is taken by the next tile as
- * soon as the wave ends, where the waves of a larger workgroup hold their slots until its slowest wave
- * ends. Included by trace_exact.hip and trace_fast.hip, which differ only in their contraction rule. */
+/* artful-hip — the tracing kernel: a workgroup of one wave traces every ray of a few neighbouring pixels,
+ * a pixel's side^2 rays in side^2 adjacent lanes (an 8 x 8 tile at one ray a pixel, 4 x 4 pixels at 2 x 2
+ * rays, 2 x 2 at 4 x 4, one pixel at 8 x 8), then sums each pixel's rays across its lanes in the order
+ * lensTile adds them and writes the pixel's mean (lens.inc:355-383): 12 bytes a pixel, whatever the rays.
+ * A wave is still 64 neighbouring rays, as many steps long as its longest, and rays end at their own
+ * step: a wave's slot on its SIMD is taken by the next tile as soon as the wave ends, where the waves of
+ * a larger workgroup would hold their slots until its slowest wave ends. Included by trace_exact.hip and
+ * trace_fast.hip, which differ only in their contraction rule. */
 #pragma once
 
 #include "launch.h"
@@ -61,17 +64,29 @@ __device__ __forceinline__ void countSteps(unsigned long long* counters, int ste
     }
 }
 
+/* The pixels of a wave: side^2 lanes a pixel, 64 / side^2 pixels (as a tileW x tileH block; with 3, 5,
+ * 6 or 7 rays a side some lanes are left over and trace nothing). */
+struct PixelTile {
+    int w, h;
+};
+
+__host__ __device__ constexpr PixelTile pixelTileOf(int side) {
+    return side == 1 ? PixelTile{8, 8} : side == 2 ? PixelTile{4, 4} : side == 3 ? PixelTile{7, 1} : side == 4 ? PixelTile{2, 2} : side == 5 ? PixelTile{2, 1} : PixelTile{1, 1};
+}
+
 template <int Precision>
 __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
     stampStart(a.startStamp, a.spanStamp);
     const ArtfHoleFrame& f = a.frame;
-    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2], count = side * side;
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
+    const PixelTile tile = pixelTileOf(side);
     const int lane = (int)threadIdx.x;
-    const int px = (int)blockIdx.x * kTraceTile + (lane & 7);
-    const int py = a.rowBegin + (int)blockIdx.y * kTraceTile + (lane >> 3);
-    const int sub = blockIdx.z, sx = sub % side, sy = sub / side;
-    const bool valid = px < width && py < rowEnd;
+    const int pixel = lane / count, sub = lane - pixel * count; /* the lane's pixel in the tile, and its ray */
+    const int px = (int)blockIdx.x * tile.w + pixel % tile.w;
+    const int py = a.rowBegin + (int)blockIdx.y * tile.h + pixel / tile.w;
+    const int sx = sub % side, sy = sub / side;
+    const bool valid = pixel < tile.w * tile.h && px < width && py < rowEnd;
 
     TraceLaw law;
     law.maxSteps = a.maxSteps > 0 ? a.maxSteps : 192;
@@ -103,11 +118,24 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wav
         }
     }
     countSteps(a.counters, ray.steps, valid);
-    if (!valid) return;
-    const size_t at = (side == 1 ? 0 : (size_t)sub * width * height) + (size_t)py * width + px;
-    a.out[0][at] = ray.r;
-    a.out[1][at] = ray.g;
-    a.out[2][at] = ray.b;
+    float r = ray.r, g = ray.g, b = ray.b;
+    if (count > 1) {
+        /* the pixel's rays in its lanes: r = r + t ray by ray from 0 (sy, then sx), then times the share,
+         * in every lane of the pixel alike (count is the same for the whole wave) */
+        const int first = pixel * count;
+        float sr = 0.f, sg = 0.f, sb = 0.f;
+        for (int k = 0; k < count; ++k) {
+            sr = sr + __shfl(r, first + k);
+            sg = sg + __shfl(g, first + k);
+            sb = sb + __shfl(b, first + k);
+        }
+        r = sr * f.image[3], g = sg * f.image[3], b = sb * f.image[3];
+    }
+    if (!valid || sub != 0) return;
+    const size_t at = (size_t)py * width + px;
+    a.out[0][at] = r;
+    a.out[1][at] = g;
+    a.out[2][at] = b;
 }
 
 inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHipTraceArgs& a, hipStream_t stream) {
@@ -115,7 +143,9 @@ inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHi
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
     const int rows = rowEnd - a.rowBegin;
     if (width <= 0 || rows <= 0 || a.rowBegin < 0 || side < 1 || side > 8) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((width + kTraceTile - 1) / kTraceTile), (unsigned)((rows + kTraceTile - 1) / kTraceTile), (unsigned)(side * side));
+    const PixelTile tile = pixelTileOf(side);
+    const dim3 grid((unsigned)((width + tile.w - 1) / tile.w), (unsigned)((rows + tile.h - 1) / tile.h));
+    if (grid.y > 65535u) return hipErrorInvalidValue;
     hipLaunchKernelGGL(kernel, grid, dim3(kTraceGroup), 0, stream, a);
     return hipGetLastError();
 }

@@ -5,10 +5,10 @@
  * and reads what they read of a scene (the map, uploaded when mapVersion changes; the fluid field, when
  * fluidVersion changes; the frame's uniforms from artfHoleFrame), through the calls of hip_scene.h, so
  * that this file compiles against this library's scene code or the reference's. Per frame it runs trace
- * -> resolve with bloom level 0 -> the rest of the bloom -> tone map (artfHipDrawFrame), its stages
- * timed by the GPU's clock, which the kernels stamp as they start. Frames take turns with two sets of
- * rays' planes, counters, HDR and pyramid, and each set has its own stream for its traces; the
- * resolves, blooms and tone maps go on a third. So a frame's trace starts as the last one's drains,
+ * (each pixel's rays summed in its lanes) -> bloom level 0 -> the rest of the bloom -> tone map
+ * (artfHipDrawFrame), its stages timed by the GPU's clock, which the kernels stamp as they start. Frames
+ * take turns with two sets of counters, HDR and pyramid, and each set has its own stream for its
+ * traces; the blooms and tone maps go on a third. So a frame's trace starts as the last one's drains,
  * and a frame's post passes run beside the next frame's trace. A set is used again only after the
  * frame before last is done with it (its stream waits for that frame's tone map when it has not ended
  * yet), and the map and fluid are replaced only after the last frame's trace (the stream that uploads
@@ -20,8 +20,8 @@
  * backend's frames in flight do (vulkan.c vkDraw waits on a fence some frames old). A device fault in
  * a queued frame is reported by the call that waits for it.
  *
- * Device memory per renderer: the map (11.5 MB) and fluid (1 MB) once; per size, the HDR planes
- * (12 B/px), the rays' planes when a pixel has several (12 B/ray), the bloom pyramid, and BGRA. */
+ * Device memory per renderer: the map (11.5 MB) and fluid (1 MB) once; per size, two sets of HDR planes
+ * (12 B/px) and bloom pyramids, and BGRA. */
 #include "render/rendererInternal.h"
 
 #include <stdio.h>
@@ -36,7 +36,7 @@ typedef struct HipRenderer {
     ArtfRenderer base;
     int device;
     void* traces[2]; /* per set: its frames' traces (and uploads) */
-    void* post;      /* the resolves, blooms and tone maps */
+    void* post;      /* the blooms and tone maps */
     void* handoff[2], *done[2]; /* per set: its trace ended; its post passes ended */
     bool busy[2];   /* per set: a frame's post passes may still use it */
     int set;        /* the set of the last frame */
@@ -53,8 +53,6 @@ typedef struct HipRenderer {
     /* per-size targets */
     float* hdr[2][3];
     size_t hdrPixels;
-    float* rays[2][3];
-    size_t rayCount;
     float* bloom[2][3];
     size_t bloomPixels;
     uint32_t* bgra;
@@ -136,10 +134,10 @@ static void hipSetOverlay(ArtfRenderer* r, const ArtfOverlay* o) {
 
 static bool settle(HipRenderer* h);
 
-static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
+static bool ensure(HipRenderer* h, uint32_t bloomPixels) {
     const size_t pixels = (size_t)h->width * h->height;
     /* a queued frame may still use what is about to be replaced */
-    const bool replace = pixels != h->hdrPixels || (rays > pixels && rays > h->rayCount) || bloomPixels > h->bloomPixels;
+    const bool replace = pixels != h->hdrPixels || bloomPixels > h->bloomPixels;
     if (replace && h->queued && !settle(h)) return false;
     if (pixels != h->hdrPixels) {
         freePlanes(h->hdr[0]), freePlanes(h->hdr[1]);
@@ -150,12 +148,6 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
         h->bgra = artfHipMalloc(pixels * 4);
         if (!h->bgra) return ok(h, 2, "allocate");
         h->hdrPixels = h->bgraPixels = pixels;
-    }
-    if (rays > pixels && rays > h->rayCount) {
-        freePlanes(h->rays[0]), freePlanes(h->rays[1]);
-        h->rayCount = 0;
-        if (!allocPlanes(h, h->rays[0], rays) || !allocPlanes(h, h->rays[1], rays)) return false;
-        h->rayCount = rays;
     }
     if (bloomPixels > h->bloomPixels) {
         freePlanes(h->bloom[0]), freePlanes(h->bloom[1]);
@@ -168,7 +160,7 @@ static bool ensure(HipRenderer* h, uint32_t bloomPixels, size_t rays) {
         const size_t bytes = 2 * ARTF_HIP_COUNTER_SLOTS * sizeof(unsigned long long);
         h->counters[k] = artfHipMalloc(bytes);
         if (!h->counters[k]) return ok(h, 2, "allocate");
-        /* zero once: each frame's resolve leaves them zero for the set's next frame */
+        /* zero once: each frame's bloom level 0 leaves them zero for the set's next frame */
         if (!ok(h, artfHipMemset(h->counters[k], 0, bytes, h->traces[k]), "clear") || !ok(h, artfHipStreamSync(h->traces[k]), "clear")) return false;
     }
     return true;
@@ -270,7 +262,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
                 h->width, h->height, side * side);
         return false;
     }
-    if (!ensure(h, bloomPixels, rays)) return false;
+    if (!ensure(h, bloomPixels)) return false;
     /* the other set than the last frame's, once the frame before last is done with it */
     const int set = h->set ^ 1;
     if (!syncScene(h, &view, set)) return false;
@@ -291,8 +283,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
         if (done < 0) return ok(h, 1, "frame order");
         if (!done && !ok(h, artfHipStreamWait(h->traces[set], h->done[set]), "frame order")) return false;
     }
-    float** out = side == 1 ? h->hdr[set] : h->rays[set];
-    for (int c = 0; c < 3; ++c) a->out[c] = out[c], fa.hdr[c] = h->hdr[set][c], fa.bloom[c] = h->bloom[set][c];
+    for (int c = 0; c < 3; ++c) a->out[c] = fa.hdr[c] = h->hdr[set][c], fa.bloom[c] = h->bloom[set][c];
     a->counters = h->counters[set];
     a->precision = h->tracing.precision;
     a->maxSteps = h->tracing.maxSteps;
@@ -329,13 +320,12 @@ static bool settle(HipRenderer* h) {
     h->busy[0] = h->busy[1] = false;
     ArtfHoleStats* st = &h->holeStats;
     /* each stage from its kernel's start to the next one's (the tone map: to its last workgroup's end);
-     * with several rays a pixel the resolve and bloom level 0 are one pass, reported as the resolve */
+     * the trace sums each pixel's rays itself, so there is no resolve to report */
     const unsigned long long* r = h->report;
     const double ms = 1e3 / h->clockHz;
-    const bool resolved = h->frame.size[2] > 1;
     st->traceMs = (double)(r[3] - r[2]) * ms;
-    st->resolveMs = resolved ? (double)(r[4] - r[3]) * ms : 0;
-    st->bloomMs = (double)(r[5] - r[resolved ? 4 : 3]) * ms;
+    st->resolveMs = 0;
+    st->bloomMs = (double)(r[5] - r[3]) * ms;
     st->toneMs = (double)(r[6] - r[5]) * ms;
     st->uploadMs = 0;
     st->rays = h->frameRays;
@@ -405,7 +395,7 @@ static void hipDestroy(ArtfRenderer* r) {
         if (h->traces[k]) artfHipStreamSync(h->traces[k]);
     if (h->post) artfHipStreamSync(h->post);
     for (int k = 0; k < 2; ++k) {
-        freePlanes(h->hdr[k]), freePlanes(h->bloom[k]), freePlanes(h->rays[k]);
+        freePlanes(h->hdr[k]), freePlanes(h->bloom[k]);
         artfHipFree(h->counters[k]);
         artfHipEventDestroy(h->handoff[k]), artfHipEventDestroy(h->done[k]);
     }

@@ -44,11 +44,25 @@ def workload_scene(native, samples=SAMPLES):
     return desc, cam
 
 
-def time_frames(native, renderer, scene, precision, steps, warmup):
+def settle_frames(renderer, scene, seconds):
+    """Frames drawn untimed for `seconds` of wall time: the frame path's first few dozen frames after an
+    idle spell run ~10% slower on the GPU's own clock (measured: 0.98 against 0.885 ms a frame at 1080p,
+    the same after 2 s of idle), so the timed frames follow a settled GPU. Returns how many were drawn."""
+    n, t_end = 0, time.perf_counter() + seconds
+    while time.perf_counter() < t_end:
+        if not renderer.draw(scene):
+            raise RuntimeError("draw failed")
+        n += 1
+    renderer.hole_stats()
+    return n
+
+
+def time_frames(native, renderer, scene, precision, steps, warmup, settle_s=0.0):
     """K frames back to back (artfRendererDraw queues a frame; the last one is waited for by the stats
-    read after the loop, inside the timed region), then frames one at a time for each stage's GPU time
-    and the step counts."""
+    read after the loop, inside the timed region), after `settle_s` seconds of untimed frames and then W
+    untimed warm-up frames; then frames one at a time for each stage's GPU time and the step counts."""
     renderer.set_tracing(precision)
+    settled = settle_frames(renderer, scene, settle_s) if settle_s > 0 else 0
     for _ in range(warmup):
         if not renderer.draw(scene):
             raise RuntimeError("draw failed")
@@ -59,6 +73,9 @@ def time_frames(native, renderer, scene, precision, steps, warmup):
             raise RuntimeError("draw failed")
     st = renderer.hole_stats()  # waits for the last frame
     wall = time.perf_counter() - t0
+    # the same frames on the GPU's own clock: the first one's trace start to the last one's tone-map end
+    assert st.frames == steps, (st.frames, steps)
+    gpu_span = st.spanMs
     trace, post, slots, nsteps = [], [], 0, 0
     staged = max(3, min(steps, 10))
     for _ in range(staged):
@@ -69,14 +86,14 @@ def time_frames(native, renderer, scene, precision, steps, warmup):
         post.append(st.resolveMs + st.bloomMs + st.toneMs)
         nsteps += st.steps
         slots += st.laneSlots
-    return {"wall_s": wall, "trace_ms": float(np.mean(trace)), "post_ms": float(np.mean(post)), "rays": int(st.rays),
+    return {"wall_s": wall, "gpu_span_ms": gpu_span, "settle": {"seconds": settle_s, "frames": settled}, "trace_ms": float(np.mean(trace)), "post_ms": float(np.mean(post)), "rays": int(st.rays),
             "steps_per_frame": nsteps / staged, "active_lanes": nsteps / max(slots, 1), "bloom_levels": st.bloomLevels}
 
 
 def kernel_ms(native, scene, precision, launches=20, warmup=100):
     """The trace kernel alone on the workload's frame: artfHipTrace launched back to back on a stream of
     its own, bracketed by two HIP events on that stream; milliseconds a launch. The kernel writes the
-    rays' planes and counts its steps, as in a frame. The untimed launches first (about 0.1 s) bring the
+    HDR planes (each pixel's rays summed in its lanes) and counts its steps, as in a frame. The untimed launches first (about 0.1 s) bring the
     GPU's clock up from where an idle spell left it."""
     import ctypes as C
     from artful_amd import abi
@@ -87,9 +104,8 @@ def kernel_ms(native, scene, precision, launches=20, warmup=100):
     a.frame64 = scene.frame64(WIDTH, HEIGHT)
     a.precision = abi.PRECISION[precision]
     host = [np.ascontiguousarray(scene.map()), np.ascontiguousarray(scene.fluid()[0])]
-    rays = WIDTH * HEIGHT * f.size[2] ** 2
     counters = 2 * 1024 * 8  # 2 x ARTF_HIP_COUNTER_SLOTS words
-    held = [L.artfHipMalloc(h.nbytes) for h in host] + [L.artfHipMalloc(rays * 4) for _ in range(3)] + [L.artfHipMalloc(counters)]
+    held = [L.artfHipMalloc(h.nbytes) for h in host] + [L.artfHipMalloc(WIDTH * HEIGHT * 4) for _ in range(3)] + [L.artfHipMalloc(counters)]
     stream, start, stop = L.artfHipStreamCreate(), L.artfHipEventCreate(), L.artfHipEventCreate()
     try:
         if not all(held) or not (stream and start and stop):
@@ -220,7 +236,7 @@ def per_view_tiles(native, views, trc, reps=3):
         a = abi.HipTraceArgs()
         a.frame, a.frame64, a.map, a.fluid, a.counters = f, f64, dmap, dmap, counters
         for c in range(3):
-            buf = L.artfHipMalloc(w * h * desc.samples ** 2 * 4)
+            buf = L.artfHipMalloc(w * h * 4)
             planes.append(buf)
             a.out[c] = buf
         a.precision, a.maxSteps, a.escapeRadius, a.disc, a.emissivity = trc.precision, trc.maxSteps, trc.escapeRadius, trc.disc, trc.emissivity
@@ -461,6 +477,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50, help="untimed frames first (50: ~60 ms, the GPU's clock up to speed)")
+    ap.add_argument("--settle", type=float, default=0.3,
+                    help="seconds of untimed frames before the warm-up frames (the frame path's first frames after idle run slow)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames and the FP32 kernel timing (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
@@ -503,13 +521,14 @@ def main():
     # (first, so the frames below start with the GPU's clock up from the idle of setting up)
     kernel = None if args.profile else {p: kernel_ms(native, scene, p) for p in ("fp32", "fp32_fast", "mixed", "fp64")}
     log(f"kernel ms: {kernel}")
-    fp32 = time_frames(native, renderer, scene, "fp32", args.steps, args.warmup)
+    fp32 = time_frames(native, renderer, scene, "fp32", args.steps, args.warmup, args.settle)
     rays = fp32["rays"]
     value = rays * args.steps / fp32["wall_s"]
     log(f"fp32: {fp32}")
     if args.profile:
         ms = kernel_ms(native, scene, "fp32")
         print(json.dumps({"rays_per_s": value, "ms_per_step": fp32["wall_s"] / args.steps * 1e3, "trace_kernel_ms": ms,
+                          "gpu_span_ms_per_step": fp32["gpu_span_ms"] / args.steps, "frame_trace_ms": fp32["trace_ms"], "post_ms": fp32["post_ms"],
                           "achieved_tflops": fp32["steps_per_frame"] * FLOPS_PER_STEP[SPIN != 0] / (ms * 1e-3) / 1e12}))
         return
     hdr32 = renderer.read_hdr()
@@ -517,7 +536,7 @@ def main():
     others = {}
     hdrs = {}
     for p in ("fp32_fast", "mixed", "fp64"):
-        others[p] = time_frames(native, renderer, scene, p, max(3, args.steps // 4), 1)
+        others[p] = time_frames(native, renderer, scene, p, max(3, args.steps // 4), 1, args.settle)
         hdrs[p] = renderer.read_hdr()
         log(f"{p}: {others[p]}")
     ref64 = hdrs["fp64"].astype(np.float64)
@@ -543,6 +562,7 @@ def main():
         "metric": "Kerr rays/s at 1080p (FP32 vs FP64 plain, <=1e-3 rel err); adjoint/forward time",
         "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": fp32["wall_s"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "gpu_span_ms_per_step": fp32["gpu_span_ms"] / args.steps, "settle": fp32["settle"],
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "kerr-1080p-frame", "width": WIDTH, "height": HEIGHT, "rays_per_pixel": SAMPLES * SAMPLES,
                    "spin": SPIN, "observer_r": RADIUS, "inclination_deg": INCLINATION, "disc": "painted, ISCO..30M",

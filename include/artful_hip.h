@@ -71,9 +71,9 @@ typedef struct ArtfHipTraceArgs {
     ArtfHoleFrame64 frame64; /* from artfHoleFrame64: read by the MIXED and FP64 tracers */
     const float* map;        /* device, ARTF_HOLE_MAP_FLOATS */
     const float* fluid;      /* device, ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT */
-    /* device planes R, G, B. With one ray a pixel: the HDR image, width * height each. With side^2 rays
-     * a pixel: each ray's light, side^2 * width * height each, ray k of pixel p at k * width * height + p
-     * (artfHipResolve then averages them in the reference's order). */
+    /* device planes R, G, B, width * height each: the HDR image, each pixel the mean of its side^2 rays,
+     * added in the order lensTile adds them and scaled by 1/side^2 (lens.inc:364-380). (artfHipAdjoint
+     * writes each ray's light instead: see ArtfHipAdjointArgs.) */
     float* out[3];
     /* device, 2 * ARTF_HIP_COUNTER_SLOTS words, added to: the rays' steps in the first half, and the
      * lane-slots their waves spent stepping in the second (steps / slots is the stepping loop's
@@ -98,7 +98,8 @@ typedef struct ArtfHipTraceArgs {
 int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
 
 /* The mean of each pixel's side^2 rays, summed in their order and scaled by 1/side^2, as lensTile
- * does (lens.inc:364-380). rays: the planes artfHipTrace wrote; hdr: width * height each. */
+ * does (lens.inc:364-380): for planes of each ray's light (ray k of pixel p at k * width * height + p),
+ * as a batch's views and the adjoint write them; hdr: width * height each. */
 int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* const hdr[3], void* stream);
 
 /* ---- batches of views (not in the reference, which draws one view a frame): many views of any
@@ -220,7 +221,8 @@ int artfHipFluidField(const ArtfHipFluid* fluid, void* stream);
 typedef struct ArtfHipAdjointArgs {
     /* the ray law and buffers, as for artfHipTrace: disc must be ARTF_DISC_ANALYTIC, precision FP32 or
      * FP64, the frame's stars 0 and its eye at rest (boost w 1), maxSteps at most 2048. out[] receives
-     * the rays' light, as artfHipTrace writes it. */
+     * each ray's light: side^2 planes of width * height (ray k of pixel p at k * width * height + p; one
+     * ray a pixel, the image); artfHipResolve averages them. */
     ArtfHipTraceArgs trace;
     const float* seed[3]; /* device: dL/d(HDR) of each pixel, R G B planes of width*height */
     /* device: 32 doubles per workgroup (artfHipAdjointBlocks of them), each workgroup's sum over its
@@ -243,15 +245,15 @@ int artfHipBloom(const ArtfHoleFrame* frame, float* const hdr[3], float* const b
 int artfHipToneMap(const ArtfHoleFrame* frame, float* const hdr[3], float* const bloom[3], uint32_t* bgra, void* stream);
 
 /* One frame of the hole, as the reference's drawHole and lensFill make it (src/software/software.c:409-427,
- * lens.inc:364-440): artfHipTrace, then artfHipResolve, artfHipBloom and artfHipToneMap, with the same
- * bytes out, in fewer launches and with nothing else on the stream: the resolve and bloom level 0 are
- * one pass (the rays' planes read once), the pyramid's small levels one workgroup's, and the frame's
- * last kernel sums the trace's counters into the report and clears them (no fill, no copy), while
- * each stage's kernel stamps the GPU's clock into the report (no events between kernels). */
+ * lens.inc:355-440): artfHipTrace (each pixel's rays summed in its lanes), then artfHipBloom and
+ * artfHipToneMap, with the same bytes out, in fewer launches and with nothing else on the stream: the
+ * pyramid's small levels are one workgroup's, bloom level 0's kernel sums the trace's counters into the
+ * report and clears them (no fill, no copy), and each stage's kernel stamps the GPU's clock into the
+ * report (no events between kernels). */
 #define ARTF_HIP_REPORT_WORDS 7
 typedef struct ArtfHipFrameArgs {
-    /* out[]: the rays' planes (side > 1) or the HDR planes (side 1); whole frames (rowBegin 0, rowEnd 0
-     * or the height); startStamp: set by artfHipDrawFrame (spanStamp is kept as given). counters: required with a report, and zero
+    /* out[]: the HDR planes, hdr[] below; whole frames (rowBegin 0, rowEnd 0 or the height); startStamp:
+     * set by artfHipDrawFrame (spanStamp is kept as given). counters: required with a report, and zero
      * before the first frame (each frame leaves them zero). */
     ArtfHipTraceArgs trace;
     float* hdr[3];   /* device, width * height each */
@@ -259,8 +261,8 @@ typedef struct ArtfHipFrameArgs {
     uint32_t* bgra;  /* device, width * height words */
     /* NULL, or host memory from artfHipHostMalloc, ARTF_HIP_REPORT_WORDS words, there once the streams are
      * synchronised: [0] the frame's steps and [1] lane-slots (the sums of the counters' halves); the GPU's
-     * clock when [2] the trace started, [3] the resolve with level 0 started (with one ray a pixel: level
-     * 0), [4] the rest of the bloom started (with no more levels: the tone map), [5] the tone map started,
+     * clock when [2] the trace started, [3] bloom level 0 started, [4] the rest of the bloom started (with
+     * no more levels: the tone map), [5] the tone map started,
      * [6] the tone map's last workgroup ended. Frames queued one after another share it: it holds the
      * last one's. */
     unsigned long long* report;

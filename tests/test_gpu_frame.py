@@ -1,7 +1,8 @@
 """One frame in one call (artfHipDrawFrame) and the bloom pyramid, against the oracle, bit for bit.
 
-artfHipDrawFrame stands for artfHipTrace, artfHipResolve, artfHipBloom and artfHipToneMap in a row (the
-reference's drawHole and lensFill, src/software/software.c:409-427, lens.inc:364-440): its HDR, its
+artfHipDrawFrame stands for artfHipTrace (which sums each pixel's rays in its lanes), artfHipBloom and
+artfHipToneMap in a row (the reference's drawHole and lensFill, src/software/software.c:409-427,
+lens.inc:355-440): its HDR, its
 bloom pyramid and its BGRA must equal the oracle's for the same frame, and the step counts it leaves in
 host memory must be the sums of the counters artfHipTrace fills for that frame, which it leaves zero.
 The pyramid's small end is built by one workgroup in LDS: the sizes below put its first level at 0,
@@ -104,13 +105,11 @@ FRAMES = [("config1", 320, 180), ("config2_small", 192, 108), ("moving_eye", 101
 
 
 def frame_args(L, dev, s, w, h, f, bloom_pixels):
-    side = f.size[2]
     a = abi.HipFrameArgs()
     a.trace.frame = f
     a.trace.map, a.trace.fluid = dev.upload(s.map()), dev.upload(s.fluid()[0])
     hdr = dev.planes(w * h)
-    out = hdr if side == 1 else dev.planes(w * h * side * side)
-    for c, p in enumerate(out):
+    for c, p in enumerate(hdr):
         a.trace.out[c] = p
     for c, (ph, pb) in enumerate(zip(hdr, dev.planes(bloom_pixels))):
         a.hdr[c], a.bloom[c] = ph, pb
@@ -260,4 +259,35 @@ def test_renderer_hole_removed_and_restored_between_queued_frames(gpu, native, s
         assert r.draw(s)
     np.testing.assert_array_equal(r.read_pixels(), want)
     r.close()
+    s.close()
+
+
+@pytest.mark.parametrize("side", range(1, 9))
+def test_trace_every_side_equals_oracle(side, gpu, dev, native, oracle):
+    """artfHipTrace at 1..8 rays a side (a wave holds 64 / side^2 pixels: 8 x 8, 4 x 4, 7 x 1, 2 x 2, 2 x 1
+    and one pixel, with lanes left over at 3, 5, 6 and 7), on a frame that is no multiple of any of those
+    tiles: each pixel's mean of its rays equals the oracle's, bit for bit, with a shutter (each ray its
+    own moment) and a moving eye; and the step counts are the oracle's."""
+    from tests.golden import cases
+    case = dict(cases.CASES["moving_eye"])
+    case["desc"] = dict(case["desc"], samples=side)
+    s = native.Scene(*cases.make(case))
+    w, h = 23, 13
+    f, _ = s.frame(w, h)
+    assert f.size[2] == side
+    L = gpu.lib()
+    a = abi.HipTraceArgs()
+    a.frame = f
+    a.map, a.fluid = dev.upload(s.map()), dev.upload(s.fluid()[0])
+    out = dev.planes(w * h)
+    for c, p in enumerate(out):
+        a.out[c] = p
+    a.counters = dev.upload(np.zeros(2 * SLOTS, np.uint64))
+    print(f"trace side {side}", flush=True)
+    assert L.artfHipTrace(C.byref(a), None) == 0 and L.artfHipStreamSync(None) == 0
+    got = np.stack([dev.read(out[c], w * h) for c in range(3)]).reshape(3, h, w)
+    want, steps = oracle.trace(f, s.map(), s.fluid()[0], 32, steps=True)
+    np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+    slots = dev.read(a.counters, 2 * SLOTS, np.uint64)
+    assert int(slots[:SLOTS].sum()) == int(steps.astype(np.int64).sum())
     s.close()
