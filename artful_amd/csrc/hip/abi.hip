@@ -1,6 +1,7 @@
 /* artful-hip — the C ABI over the kernels (include/artful_hip.h). Every entry checks what it is given
  * on the host before anything is launched, and returns a HIP status; nothing here allocates except
- * artfHipMalloc. */
+ * artfHipMalloc (and artfHipHostMalloc, the streams, events and graphs it is asked for). */
+#include <pthread.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -197,6 +198,79 @@ int artfHipDrawFrame(const ArtfHipFrameArgs* a, void* stream) {
         if (!a->hdr[c] || !a->bloom[c] || a->trace.out[c] != a->hdr[c]) return (int)hipErrorInvalidValue;
     if (a->postStream && (!a->handoff || a->postStream == stream)) return (int)hipErrorInvalidValue;
     return (int)launchFrame(*a, fresh(stream));
+}
+
+} // extern "C"
+
+/* ---- artfHipDrawFrameStatic: artfHipDrawFrame with its scratch in a static device array ----
+ *
+ * A frame's only scratch is the trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, 16 KiB): the
+ * trace adds to them, bloom level 0's first workgroup sums them into the report and zeroes them. Here
+ * they live in `scratch`, one copy per GPU for as long as the process runs, so that a caller that may not
+ * allocate (under graph capture, in a serving loop) can still draw frames. Its size does not depend on
+ * the frame: a slot is a workgroup's index modulo ARTF_HIP_COUNTER_SLOTS. In host code `scratch` names a
+ * host-side stand-in, never device memory: the device address comes from hipGetSymbolAddress, once per
+ * device with that device current, and only that address is handed to kernels. */
+static __device__ __align__(256) unsigned long long scratch[2 * ARTF_HIP_COUNTER_SLOTS];
+
+static constexpr int kScratchDevices = 64;
+static unsigned long long* scratchAt[kScratchDevices];
+static bool scratchKnown[kScratchDevices];
+static pthread_mutex_t scratchLock = PTHREAD_MUTEX_INITIALIZER;
+
+/* The current device's copy of `scratch` (NULL, with *status set, if it cannot be had). */
+static unsigned long long* scratchOfCurrent(hipError_t* status) {
+    int device = 0;
+    if ((*status = hipGetDevice(&device)) != hipSuccess) return nullptr;
+    if (device < 0 || device >= kScratchDevices) {
+        *status = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    pthread_mutex_lock(&scratchLock);
+    if (!scratchKnown[device]) {
+        void* at = nullptr;
+        *status = hipGetSymbolAddress(&at, HIP_SYMBOL(scratch));
+        if (*status == hipSuccess && at) {
+            scratchAt[device] = (unsigned long long*)at;
+            scratchKnown[device] = true;
+        } else if (*status == hipSuccess) {
+            *status = hipErrorInvalidSymbol;
+        }
+    }
+    unsigned long long* at = scratchKnown[device] ? scratchAt[device] : nullptr;
+    pthread_mutex_unlock(&scratchLock);
+    return at;
+}
+
+/* Zeroes the counters by a kernel (no memset node: the call may be captured into a graph). */
+__global__ __launch_bounds__(256) void clearScratchKernel(unsigned long long* at) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < 2 * ARTF_HIP_COUNTER_SLOTS) at[i] = 0ull;
+}
+
+extern "C" {
+
+int artfHipDrawFrameStatic(const ArtfHipFrameArgs* a, void* stream) {
+    if (!a) return (int)hipErrorInvalidValue;
+    ArtfHipFrameArgs b = *a;
+    b.trace.counters = nullptr; /* whatever was given: the static array stands in for them */
+    if (!traceFits(&b.trace) || !pyramidFits(&b.trace.frame) || !b.bgra) return (int)hipErrorInvalidValue;
+    const ArtfHoleFrame* f = &b.trace.frame;
+    if (b.trace.rowBegin != 0 || (b.trace.rowEnd != 0 && b.trace.rowEnd < (int)f->size[1])) return (int)hipErrorInvalidValue;
+    for (int c = 0; c < 3; ++c)
+        if (!b.hdr[c] || !b.bloom[c] || b.trace.out[c] != b.hdr[c]) return (int)hipErrorInvalidValue;
+    if (b.postStream) return (int)hipErrorInvalidValue; /* one stream: a frame's counters are read before the next frame's trace adds */
+    if (b.report) {
+        hipError_t status;
+        unsigned long long* at = scratchOfCurrent(&status);
+        if (!at) return (int)status;
+        hipStream_t s = fresh(stream);
+        hipLaunchKernelGGL(clearScratchKernel, dim3((2 * ARTF_HIP_COUNTER_SLOTS + 255) / 256), dim3(256), 0, s, at);
+        if ((status = hipGetLastError()) != hipSuccess) return (int)status;
+        b.trace.counters = at;
+        return (int)launchFrame(b, s);
+    }
+    return (int)launchFrame(b, fresh(stream));
 }
 
 static bool batchFits(const ArtfHipBatchArgs* a) {

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         "artfHipHostMalloc": ([C.c_size_t], P), "artfHipHostFree": ([P], None),
         "artfHipDownloadAsync": ([P, P, C.c_size_t, P], I),
         "artfHipDrawFrame": ([C.POINTER(abi.HipFrameArgs), P], I),
+        "artfHipDrawFrameStatic": ([C.POINTER(abi.HipFrameArgs), P], I),
         "artfHipEventCreateOrder": ([], P), "artfHipStreamWait": ([P, P], I), "artfHipEventDone": ([P], I),
         "artfHipTrace": ([C.POINTER(abi.HipTraceArgs), P], I),
         "artfHipResolve": ([C.POINTER(abi.HoleFrame), C.c_void_p * 3, C.c_void_p * 3, P], I),

@@ -280,6 +280,17 @@ typedef struct ArtfHipFrameArgs {
 
 int artfHipDrawFrame(const ArtfHipFrameArgs* args, void* stream);
 
+/* artfHipDrawFrame for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc); it has no counterpart in the reference. The same
+ * frame, bytes and report, with the trace's step counters (args->trace.counters is ignored and may be
+ * NULL) in a static device array of 2 * ARTF_HIP_COUNTER_SLOTS words (16 KiB), one per GPU, held until
+ * the process ends, whatever the frame's size; frames are limited as artfHipDrawFrame's are (at most
+ * 65535 rows, fewer than 2^31 rays), others refused with hipErrorInvalidValue before any launch. It
+ * calls no allocator. The whole frame runs on `stream` (postStream must be NULL: refused otherwise).
+ * Every call on one GPU shares that GPU's array, zeroed by a kernel at each call's start: calls that
+ * overlap on one GPU (other streams, other threads) race on it; calls in order on one stream do not. */
+int artfHipDrawFrameStatic(const ArtfHipFrameArgs* args, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
