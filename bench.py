@@ -151,36 +151,42 @@ def traffic_from_profiles():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(native, seconds=15.0):
-    """The reference's own software renderer (oracle/_ref), or the oracle where it is absent, on a
-    band of the same frame: rays/s of its trace + bloom (the reference's lensMs)."""
+def cpu_baseline(native, frames=12, warmup=2):
+    """The reference's own software renderer (oracle/_ref, built from its sources) on the workload's
+    frame itself: 1920x1080 at 2x2 rays a pixel, the same view, min(CPUs, 16) threads (the reference's
+    rule, src/core/threads.c:54, 144), AVX2; the median over `frames` frames after `warmup`, of its
+    lensMs (trace + bloom, software.c:673-675) and of its rasterMs (the tiles' pass, whose work with no
+    triangles is the tone map, kernels.inc:1616-1617). It traces the very rays the GPU does (its HDR is
+    the GPU's bit for bit), so the steps a ray are the GPU frame's. Where the reference library is absent,
+    the oracle (this project's C restatement, bit-equal to it) is timed on a band of rows of the same
+    frame instead."""
     from oracle import oracle as O
     desc, cam = workload_scene(native)
     cores = os.cpu_count() or 1
     threads = min(len(os.sched_getaffinity(0)), 16)
+    rays = WIDTH * HEIGHT * SAMPLES * SAMPLES
     if O.reference_available():
+        import ctypes as C
         L = O.reference()
         scene = O.RefScene(desc, cam)
-        # a band of the frame, at its full width and ray density: 1920 x 128, 2x2 rays a pixel
-        band_h = 128
-        r = L.refRendererNew(WIDTH, band_h, 0, threads)
-        times = []
-        t_end = time.perf_counter() + seconds
-        import ctypes as C
+        r = L.refRendererNew(WIDTH, HEIGHT, 0, threads)
         lens, raster, cpu = C.c_double(), C.c_double(), C.c_double()
-        while time.perf_counter() < t_end or len(times) < 3:
+        got = []
+        for k in range(warmup + frames):
             L.refRendererDraw(r, scene.ptr)
             L.refRendererStats(r, C.byref(lens), C.byref(raster), C.byref(cpu))
-            times.append(lens.value)
-            if len(times) >= 30:
-                break
+            if k >= warmup:
+                got.append((lens.value, raster.value, cpu.value))
+        description = L.refRendererDescription(r).decode()
         L.refRendererFree(r)
         scene.close()
-        ms = float(np.median(times[1:] if len(times) > 2 else times))
-        rays = WIDTH * band_h * SAMPLES * SAMPLES
-        return {"value": rays / (ms / 1e3), "unit": "rays/s", "cores": threads, "kind": "reference",
-                "sample": f"reference software renderer (AVX2, {threads} threads of {cores} CPUs): {WIDTH}x{band_h} band of the "
-                          f"same view at 2x2 rays/pixel, trace+bloom (lensMs), median of {len(times) - 1} frames"}
+        lens_ms, raster_ms, cpu_ms = (float(np.median([g[i] for g in got])) for i in range(3))
+        return {"value": rays / (lens_ms / 1e3), "unit": "rays/s", "cores": threads, "kind": "reference",
+                "sample": f"reference software renderer ({description}; {threads} threads of {cores} CPUs): the whole "
+                          f"{WIDTH}x{HEIGHT} frame of the same view at 2x2 rays/pixel, trace+bloom (lensMs), median of {frames} "
+                          f"frames after {warmup}",
+                "lens_ms": lens_ms, "tone_ms": raster_ms, "frame_ms": cpu_ms,
+                "rays_per_s_with_tone_map": rays / ((lens_ms + raster_ms) / 1e3)}
     scene = native.Scene(desc, cam)
     frame, _ = scene.frame(WIDTH, HEIGHT)
     m, (fl, _, _) = scene.map(), scene.fluid()
@@ -188,8 +194,8 @@ def cpu_baseline(native, seconds=15.0):
     t0 = time.perf_counter()
     O.trace(frame, m, fl, 32, rows=band, threads=threads)
     dt = time.perf_counter() - t0
-    rays = WIDTH * (band[1] - band[0]) * SAMPLES * SAMPLES
-    return {"value": rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+    band_rays = WIDTH * (band[1] - band[0]) * SAMPLES * SAMPLES
+    return {"value": band_rays / dt, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"oracle restatement, {threads} threads: rows {band[0]}..{band[1]} of the same frame, 2x2 rays/pixel, trace only"}
 
 
