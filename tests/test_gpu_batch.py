@@ -176,11 +176,17 @@ def test_c_abi_rejects_bad_schedules(gpu):
     assert L.artfHipTraceViews(a, None) != 0  # no views
 
 
-def test_huge_frame_in_slices(gpu, renderer):
+def test_huge_frame_in_slices(gpu, renderer, oracle):
     """A frame of 8x8 rays a pixel (640 x 360: 14.7 M rays) as a batch of one view in slices of 20,000
-    tiles (12 launches), against the renderer's one launch."""
+    tiles (12 launches), against the renderer's one launch, and bands of its rows against the oracle."""
     view = view_of(.998, 60, 60, 30, 640, 360, fov_y=.35, samples=8)
     tracing = dict(precision="fp32", disc="painted")
     images, stats = batch_images(gpu, [view], schedule=(0, 2, 64, 8, 20000), **tracing)
     assert stats.rays == 640 * 360 * 64
     assert same_bits(images[0], rendered(gpu, renderer, view, **tracing))
+    s = gpu.Scene(view[0], view[1])
+    f, _ = s.frame(640, 360)
+    for band in ((0, 2), (176, 184), (358, 360)):
+        want = oracle.trace(f, s.map(), s.fluid()[0], 32, rows=band)
+        assert same_bits(images[0][:, band[0]:band[1]], want[:, band[0]:band[1]]), band
+    s.close()
