@@ -221,7 +221,9 @@ def adjoint_fit(native, iterations=200):
             "adjoint_over_forward": float(np.median(adj) / np.median(fwd)),
             "fit": {"iterations": iterations, "wall_s": wall, "loss_first": history[0][0], "loss_last": history[-1][0],
                     "loss_best": history[best][0], "params_last": [float(v) for v in history[-1][1]],
-                    "hidden": list(K.HIDDEN), "start": list(K.START)}}
+                    "hidden": list(K.HIDDEN), "start": list(K.START),
+                    "params_error": {n: float(v - h) for n, v, h in zip(K.PARAMS, history[-1][1], K.HIDDEN)},
+                    "preconditioner": "Gauss-Newton whitening from forward central differences, refreshed every 25 iterations"}}
 
 
 def per_view_tiles(native, views, trc, reps=3):

@@ -69,24 +69,51 @@ def mse(image, target):
     return ((image - target) ** 2).mean()
 
 
-# a step size per parameter (spin, degrees, masses, masses, q): Adam moves each by about this a step
-STEP_SIZES = (.01, .5, .03, .2, .01)
+# central-difference steps of the forward renders the preconditioner is built from (spin, degrees, masses, masses, q)
+FD_STEPS = (2e-3, .1, 5e-3, .02, 2e-3)
 
 
-def fit(renderer, target: np.ndarray, iterations=200, start=START, steps=STEP_SIZES):
-    """Adam on the five parameters, from start toward the image target; the loss and parameters at each
-    iteration. The parameters are p = start + steps * z, z optimised with unit learning rate."""
+def gauss_newton(renderer, params, steps=FD_STEPS) -> np.ndarray:
+    """The loss's Gauss-Newton matrix (2/N) J^T J at params, J = d(image)/d(params) by central differences
+    of forward renders (ten of them): the metric the fit's preconditioner whitens."""
+    cols = []
+    for i, h in enumerate(steps):
+        e = np.zeros(5)
+        e[i] = h
+        cols.append(((render(renderer, params + e).astype(np.float64) - render(renderer, params - e)) / (2 * h)).ravel())
+    J = np.stack(cols, 1)
+    return 2.0 / J.shape[0] * (J.T @ J)
+
+
+def fit(renderer, target: np.ndarray, iterations=200, start=START, lr=.01, refresh=25, floor=1e-3):
+    """Adam on the five parameters, from start toward the image target, on the adjoint's gradients; the
+    loss and parameters at each iteration.
+
+    Spin and inclination pull the image alike (the loss along spin alone is shallow, and a fit with a step
+    size per parameter stalls in the valley between them), so Adam runs in whitened coordinates: every
+    `refresh` iterations the Gauss-Newton matrix H = V diag(lam) V^T is taken at the current point
+    (eigenvalues held above `floor` x the largest), and p = p_ref + V diag(lam^-1/2) z, z optimised from 0
+    by a fresh Adam whose learning rate follows a cosine from lr down to lr / 20 over the whole fit."""
     t = torch.from_numpy(np.asarray(target, np.float64))
-    s = torch.tensor(steps, dtype=torch.float64)
-    p0 = torch.tensor(start, dtype=torch.float64)
-    z = torch.zeros(5, dtype=torch.float64, requires_grad=True)
-    opt = torch.optim.Adam([z], lr=1.0)
+    p = np.array(start, np.float64)
     history = []
-    for _ in range(iterations):
-        opt.zero_grad()
-        params = p0 + s * z
-        loss = mse(KerrImage.apply(params, renderer).double(), t)
-        loss.backward()
-        history.append((float(loss), params.detach().numpy().copy()))
-        opt.step()
+    k = 0
+    while k < iterations:
+        lam, V = np.linalg.eigh(gauss_newton(renderer, p))
+        lam = np.maximum(lam, lam.max() * floor)
+        to_params = torch.from_numpy(V @ np.diag(lam ** -.5))
+        p_ref = torch.from_numpy(p.copy())
+        z = torch.zeros(5, dtype=torch.float64, requires_grad=True)
+        opt = torch.optim.Adam([z], lr=lr)
+        for _ in range(min(refresh, iterations - k)):
+            for group in opt.param_groups:
+                group["lr"] = lr * (.05 + .95 * .5 * (1 + math.cos(math.pi * k / iterations)))
+            opt.zero_grad()
+            params = p_ref + to_params @ z
+            loss = mse(KerrImage.apply(params, renderer).double(), t)
+            loss.backward()
+            history.append((float(loss.detach()), params.detach().numpy().copy()))
+            opt.step()
+            k += 1
+        p = (p_ref + to_params @ z).detach().numpy()
     return history

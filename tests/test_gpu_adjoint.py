@@ -148,16 +148,20 @@ def test_autograd_function(ctx, problem):
     np.testing.assert_allclose(params.grad.numpy(), direct, rtol=1e-5, atol=1e-12)
 
 
-def test_fit_moves_toward_hidden(ctx, problem):
-    """Adam on the five parameters from the start point: the loss falls by half in 40 steps and every
-    parameter but the least constrained moves toward its hidden value."""
-    target, _, _ = problem
-    history = K.fit(ctx, target, iterations=40)
+def test_fit_recovers_hidden(gpu):
+    """The adjoint workload: 512x512, the image of the hidden parameters (spin .7, inclination 35, disc
+    inner 3.4, outer 18, q 2.5) fitted from the start (.3, 60, 5, 25, 3) by 200 Adam iterations on the
+    adjoint's gradients (preconditioned: tests/kerr_autograd.py): every parameter ends near its hidden
+    value (measured: .714, 34.50, 3.418, 17.96, 2.511) and the loss falls by three orders of magnitude."""
+    c = gpu.HoleAdjoint(512, 512)
+    target = K.render(c, K.HIDDEN)
+    history = K.fit(c, target, iterations=200)
+    c.close()
     first, last = history[0][0], history[-1][0]
-    print("loss", first, "->", last, "params", history[-1][1])
-    assert last < first / 2
-    moved = np.abs(history[-1][1] - np.array(K.HIDDEN)) < np.abs(np.array(K.START) - np.array(K.HIDDEN))
-    assert moved.sum() >= 4
+    error = history[-1][1] - np.array(K.HIDDEN)
+    print("loss", first, "->", last, "params", history[-1][1], "error", error)
+    assert last < first * 1e-3
+    assert np.all(np.abs(error) < np.array([.05, 1.5, .05, .2, .05])), error
 
 
 def test_adjoint_costs_at_most_four_forwards(gpu):

@@ -64,15 +64,18 @@ def test_fp64_equals_oracle64(renderer, scene_of, oracle):
     assert np.median(rel) < 1e-6 and (rel > 1e-5).mean() < 1e-3
 
 
-@pytest.mark.parametrize("precision,mean,share", [("fp32", 2e-4, .01), ("fp32_fast", 2e-4, .01), ("mixed", 2e-4, .01)])
+# measured on MI355X (mean, share past 1e-3): FP32 4.3e-5, 0.26%; FP32 with FMA 3.6e-5, 0.25%; mixed 3.1e-5, 0.17%
+@pytest.mark.parametrize("precision,mean,share", [("fp32", 9e-5, .006), ("fp32_fast", 8e-5, .005), ("mixed", 7e-5, .0035)])
 def test_precisions_against_oracle64(renderer, scene_of, oracle, precision, mean, share):
-    """Mean relative HDR error and the share of pixels past 1e-3 (floor 1e-3), painted disc."""
+    """Mean relative HDR error and the share of pixels past 1e-3 (floor 1e-3) against the FP64 oracle,
+    painted disc, each held to about twice what it measures (the maxima, from pixels whose ray changes
+    fate between precisions, are printed with the 99.9th percentile)."""
     s = scene_of("config1")
     f, _ = s.frame(320, 180)
     want = oracle.trace(f, s.map(), s.fluid()[0], 64, s.frame64(320, 180))
     hdr, _ = draw(renderer, s, 320, 180, precision=precision)
     rel = relative(hdr, want)
-    print(precision, rel.mean(), (rel > 1e-3).mean(), rel.max())
+    print(precision, "mean", rel.mean(), "share", (rel > 1e-3).mean(), "p99.9", np.quantile(rel, .999), "max", rel.max())
     assert rel.mean() < mean and (rel > 1e-3).mean() < share
 
 
@@ -96,7 +99,8 @@ def test_mixed_closer_to_fp64_than_fp32(renderer, scene_of, oracle):
 @pytest.mark.parametrize("precision", ["fp32", "fp32_fast", "mixed", "fp64"])
 def test_analytic_disc(renderer, native, oracle, precision):
     """The analytic r^-q disc on the CPU-sized view (2000 steps, escape at r > 100) against the
-    FP64 oracle: the smooth disc keeps every precision within 1e-3 on nearly all pixels."""
+    FP64 oracle: the smooth disc keeps every precision within 5e-4 on every pixel and 1.5e-6 in the mean
+    (measured: mean 6.3e-7 at most, largest 1.6e-4; FP64 equal to the oracle)."""
     desc, cam = cases.make(cases.CASES["config1"])
     desc.stars = 0.0
     s = native.Scene(desc, cam)
@@ -105,9 +109,9 @@ def test_analytic_disc(renderer, native, oracle, precision):
     want = oracle.trace(f, s.map(), s.fluid()[0], 64, s.frame64(160, 90), lw)
     hdr, _ = draw(renderer, s, 160, 90, precision=precision, disc="analytic", emissivity=3.0, max_steps=2000, escape_radius=100.0)
     rel = relative(hdr, want)
-    print(precision, rel.mean(), (rel > 1e-3).mean(), rel.max())
+    print(precision, "mean", rel.mean(), "p99.9", np.quantile(rel, .999), "max", rel.max())
     assert want.max() > 0
-    assert rel.mean() < 1e-3 and (rel > 1e-3).mean() < .02
+    assert rel.mean() < 1.5e-6 and rel.max() < 5e-4
     if precision == "fp32":
         got32 = oracle.trace(f, s.map(), s.fluid()[0], 32, None, lw)
         assert relative(hdr, got32).max() < 1e-3  # the oracle's FP32 and the GPU's share all but exp2/log2
