@@ -1,6 +1,6 @@
 """Builds artful_amd/libartful_hip.so: the host C library (scene, fluid, renderer, HIP backend)
-with gcc, the HIP kernels with hipcc for gfx950, linked by hipcc; and artful_amd/artful-blackhole,
-the headless driver.
+with gcc, the HIP kernels with hipcc for gfx950, linked by hipcc; and the headless drivers
+artful_amd/artful-blackhole and artful_amd/artful-render.
 
 Host C is built with the reference's floating-point flags (-ffp-contract=off -fno-math-errno
 -fno-trapping-math, Artful CMakeLists.txt:46-58), since the disc map, the fluid and the frame
@@ -23,6 +23,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libartful_hip.so"
 DRIVER = PKG / "artful-blackhole"
+RENDER = PKG / "artful-render"
 OBJ = ROOT / "_build" / "artful_hip"
 ARCH = os.environ.get("ARTF_OFFLOAD_ARCH", "gfx950")
 
@@ -84,8 +85,11 @@ def build(verbose: bool = False) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread", "-lm"])
     os.replace(tmp, OUT)
-    # the headless driver (tools/blackhole.c's loop), linked against the library beside it
-    _run(["gcc", *C_FLAGS, str(CSRC / "drivers" / "blackhole.c"), f"-L{PKG}", "-lartful_hip", "-Wl,-rpath,$ORIGIN", "-lm", "-o", str(DRIVER)])
+    # the headless drivers (tools/blackhole.c's loop, tools/render.c's black-hole scene), linked against the
+    # library beside them
+    for main, exe in (("blackhole.c", DRIVER), ("render.c", RENDER)):
+        _run(["gcc", *C_FLAGS, str(CSRC / "drivers" / main), str(CSRC / "drivers" / "frames.c"), f"-L{PKG}", "-lartful_hip",
+              "-Wl,-rpath,$ORIGIN", "-lm", "-o", str(exe)])
     return OUT
 
 

@@ -7,10 +7,12 @@
  *
  *   artful-blackhole [--size w h] [--seconds n | --frames n] [--step dt] [--spin a] [--samples s]
  *                    [--disc-on-device] [--no-orbit] [--doppler d] [--save k dir] [--report file]
+ *                    [--describe file]
  *
  * --step dt advances the hole's time by dt a frame (0.2 is one fluid step) instead of by the clock,
  * so that a run is the same frames every time. --disc-on-device keeps the disc's map and fluid on the
- * GPU (artfSceneSetDiscOnDevice). --save k dir writes every k-th frame as dir/frame-NNNNN.ppm. */
+ * GPU (artfSceneSetDiscOnDevice). --save k dir writes every k-th frame as dir/frame-NNNNN.ppm, and
+ * --describe file a line for each of them with the hole's description and the camera that made it. */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -19,10 +21,12 @@
 #include "artful/device_disc.h"
 #include "artful/renderer.h"
 #include "artful/scene.h"
+#include "frames.h"
 
 typedef struct Flight {
     ArtfScene* scene;
     ArtfBlackHoleDesc hole;
+    ArtfCamera camera;
     float around, tilt, distance;
     bool orbiting;
 } Flight;
@@ -35,32 +39,16 @@ static void place(Flight* f) {
     const float beta = f->orbiting ? artfBlackHoleOrbitSpeed(f->hole.spin, f->distance) : 0.f;
     f->hole.velocity = artfVec3Scale(side, beta);
     const ArtfVec3 aim = artfVec3Scale(f->hole.velocity, f->distance / sqrtf(1.f - beta * beta));
-    ArtfCamera camera = {artfVec3(sinf(f->around) * c * f->distance, sinf(f->tilt) * f->distance, cosf(f->around) * c * f->distance), aim,
-                         artfVec3(0, 1, 0), .33f, .1f};
+    const ArtfCamera camera = {artfVec3(sinf(f->around) * c * f->distance, sinf(f->tilt) * f->distance, cosf(f->around) * c * f->distance), aim,
+                               artfVec3(0, 1, 0), .33f, .1f};
+    f->camera = camera;
     artfSceneSetCamera(f->scene, &camera);
-}
-
-static bool savePpm(const char* dir, int index, const uint32_t* bgra, int width, int height) {
-    char path[1024];
-    snprintf(path, sizeof path, "%s/frame-%05d.ppm", dir, index);
-    FILE* out = fopen(path, "wb");
-    if (!out) return false;
-    fprintf(out, "P6\n%d %d\n255\n", width, height);
-    unsigned char* row = malloc((size_t)width * 3);
-    for (int y = 0; y < height; ++y) {
-        for (int x = 0; x < width; ++x) {
-            const uint32_t p = bgra[(size_t)y * width + x];
-            row[x * 3] = (unsigned char)(p >> 16), row[x * 3 + 1] = (unsigned char)(p >> 8), row[x * 3 + 2] = (unsigned char)p;
-        }
-        fwrite(row, 3, (size_t)width, out);
-    }
-    free(row);
-    return fclose(out) == 0;
 }
 
 static void usage(void) {
     fprintf(stderr, "artful-blackhole [--size w h] [--seconds n | --frames n] [--step dt] [--spin a] [--samples s]\n"
-                    "                 [--disc-on-device] [--no-orbit] [--doppler d] [--save k dir] [--report file]\n");
+                    "                 [--disc-on-device] [--no-orbit] [--doppler d] [--save k dir] [--report file]\n"
+                    "                 [--describe file]\n");
 }
 
 int main(int argc, char** argv) {
@@ -70,6 +58,7 @@ int main(int argc, char** argv) {
     bool onDevice = false, orbit = true, spinGiven = false;
     const char* saveDir = NULL;
     const char* report = "artful-blackhole.txt";
+    const char* describe = NULL;
     for (int i = 1; i < argc; ++i) {
         const char* a = argv[i];
         if (!strcmp(a, "--size") && i + 2 < argc) width = atoi(argv[++i]), height = atoi(argv[++i]);
@@ -83,6 +72,7 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "--no-orbit")) orbit = false;
         else if (!strcmp(a, "--save") && i + 2 < argc) saveEvery = atoi(argv[++i]), saveDir = argv[++i];
         else if (!strcmp(a, "--report") && i + 1 < argc) report = argv[++i];
+        else if (!strcmp(a, "--describe") && i + 1 < argc) describe = argv[++i];
         else {
             usage();
             return 2;
@@ -110,6 +100,7 @@ int main(int argc, char** argv) {
     f.tilt = .06f, f.distance = 74.1f, f.orbiting = orbit;
 
     uint32_t* pixels = malloc((size_t)width * height * sizeof(uint32_t));
+    FILE* described = describe ? fopen(describe, "w") : NULL;
     const double start = artfNowMs();
     double last = start, gpuMs = 0;
     int drawn = 0;
@@ -129,8 +120,11 @@ int main(int argc, char** argv) {
         ArtfHoleStats st;
         if (artfRendererHoleStats(renderer, &st)) gpuMs += st.traceMs + st.resolveMs + st.bloomMs + st.toneMs;
         if (saveDir && saveEvery > 0 && drawn % saveEvery == 0) {
+            char path[1024];
+            snprintf(path, sizeof path, "%s/frame-%05d.ppm", saveDir, drawn);
             artfRendererReadPixels(renderer, pixels);
-            if (!savePpm(saveDir, drawn, pixels, width, height)) fprintf(stderr, "artful-blackhole: could not write into %s\n", saveDir);
+            if (!driverSaveImage(path, pixels, width, height)) fprintf(stderr, "artful-blackhole: could not write into %s\n", saveDir);
+            driverDescribe(described, drawn, &f.hole, &f.camera);
         }
         ++drawn;
         if (frames > 0 && drawn >= frames) break;
@@ -144,6 +138,7 @@ int main(int argc, char** argv) {
         fclose(out);
     }
     printf("%d frames, %.2f ms a frame\n", drawn, wall / (drawn ? drawn : 1));
+    if (described) fclose(described);
     free(pixels);
     artfSceneFree(f.scene);
     artfRendererFree(renderer);
