@@ -35,18 +35,45 @@ static void* orDie(void* p) {
     return p;
 }
 
-void* artfMalloc(size_t size) { return orDie(malloc(size ? size : 1)); }
-void* artfCalloc(size_t count, size_t size) { return orDie(calloc(count ? count : 1, size ? size : 1)); }
-void* artfRealloc(void* p, size_t size) { return orDie(realloc(p, size ? size : 1)); }
-void artfFree(void* p) { free(p); }
+/* Built with ARTFUL_COUNT_ALLOCATIONS, the blocks alive are counted (as the reference's base.c:32-40
+ * does): a test then knows that everything was given back. */
+#ifdef ARTFUL_COUNT_ALLOCATIONS
+#include <stdatomic.h>
+static atomic_long alive;
+long artfAllocationsAlive(void) { return atomic_load(&alive); }
+#define COUNT(n) atomic_fetch_add(&alive, n)
+#else
+#define COUNT(n) ((void)0)
+#endif
+
+void* artfMalloc(size_t size) {
+    COUNT(1);
+    return orDie(malloc(size ? size : 1));
+}
+void* artfCalloc(size_t count, size_t size) {
+    COUNT(1);
+    return orDie(calloc(count ? count : 1, size ? size : 1));
+}
+void* artfRealloc(void* p, size_t size) {
+    if (!p) COUNT(1);
+    return orDie(realloc(p, size ? size : 1));
+}
+void artfFree(void* p) {
+    if (p) COUNT(-1);
+    free(p);
+}
 
 void* artfAlignedMalloc(size_t size, size_t align) {
     void* p = NULL;
     if (align < sizeof(void*)) align = sizeof(void*);
     if (posix_memalign(&p, align, size ? size : 1)) p = NULL;
+    COUNT(1);
     return orDie(p);
 }
-void artfAlignedFree(void* p) { free(p); }
+void artfAlignedFree(void* p) {
+    if (p) COUNT(-1);
+    free(p);
+}
 
 double artfNowMs(void) {
     struct timespec t;

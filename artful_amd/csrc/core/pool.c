@@ -58,7 +58,13 @@ static void* workerMain(void* arg) {
     return NULL;
 }
 
+static atomic_int chosenSize;
+
+void artfPoolSetDefaultSize(int threads) { atomic_store(&chosenSize, threads > 0 ? artfClampi(threads, 1, POOL_MOST) : 0); }
+
 int artfPoolDefaultSize(void) {
+    const int chosen = atomic_load(&chosenSize);
+    if (chosen > 0) return chosen;
     cpu_set_t set;
     int n = 0;
     if (!sched_getaffinity(0, sizeof set, &set)) n = CPU_COUNT(&set);

@@ -18,5 +18,9 @@ int artfPoolSize(const ArtfPool* p);
 void artfPoolRun(ArtfPool* p, ArtfJobFn fn, void* user, uint32_t count);
 /* What artfPoolNew(0) would pick. */
 int artfPoolDefaultSize(void);
+/* threads > 0: what artfPoolNew(0) picks from now on, in this process (at most 16); 0: the rule
+ * again. The pools the scene makes for the disc's tables and fluid are new at each build, so this
+ * reaches them: the bytes they make are the same whatever the size (tested at 1, 3 and 16). */
+void artfPoolSetDefaultSize(int threads);
 
 #endif

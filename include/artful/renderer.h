@@ -55,7 +55,7 @@ typedef struct ArtfRendererOptions {
     ArtfSurface surface; /* the HIP backend draws into memory whatever is given */
     int width, height;
     bool noVsync;
-    int threads;         /* host threads for the disc's tables and fluid: 0 as many as allowed, at most 16 */
+    int threads;         /* the reference's software renderer's threads; the HIP backend draws on the GPU */
     ArtfSimd simd;
     bool antialias;
     bool hdr;
