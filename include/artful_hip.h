@@ -1,6 +1,6 @@
 /* artful-hip — the thin C ABI over the HIP kernels (gfx950). Plain C: POD arguments, device pointers
  * as plain pointers, streams as void*, an int status (0, else a HIP error number) and nothing thrown.
- * The C host backend (artful_amd/csrc/render/hip_backend.c) calls these; so may any FFI (ctypes,
+ * The C host backend (artful_amd/csrc/render/hip_renderer.c) calls these; so may any FFI (ctypes,
  * cgo, JNI) that wants the kernels without the renderer.
  *
  * Each entry replaces a piece of Artful 0.4.0's black-hole path, named beside it. */
@@ -267,12 +267,12 @@ typedef struct ArtfHipFrameArgs {
      * last one's. */
     unsigned long long* report;
     /* NULL: the whole frame on the call's stream. Else the trace alone goes on the call's stream, and the
-     * resolve, the bloom and the tone map on this second one, after `handoff` (artfHipEventCreateOrder)
-     * is recorded past the trace; `done` (the same kind, or NULL) is recorded after them. The call's
-     * stream is then free for the next frame's trace while they run: the caller keeps later frames off
-     * what they still read and write (the rays' planes, the counters, the HDR and the pyramid: a second
-     * set for the next frame, its stream waiting for `done` before a set is used again), and waits for
-     * both streams before reading the frame back. */
+     * bloom and the tone map on this second one, after `handoff` (artfHipEventCreateOrder) is recorded
+     * past the trace; `done` (the same kind, or NULL) is recorded after them. The call's stream is then
+     * free for the next frame's trace while they run: the caller keeps later frames off what they still
+     * read and write (the counters, the HDR and the pyramid: a second set for the next frame, its stream
+     * waiting for `done` before a set is used again), and waits for both streams before reading the
+     * frame back. */
     void* postStream;
     void* handoff;
     void* done;
