@@ -14,6 +14,14 @@
 
 namespace artf {
 
+/* n / d for 0 <= n < 4096 and 1 <= d <= 64, by the hardware's reciprocal: (n + 1/2) / d lies at least
+ * 1/(2d) >= 1/128 from an integer, and the product is within 4096.5 x 2^-22 (1e-3) of it, so the
+ * truncation is the quotient; 4 instructions where an integer division by a value the compiler does not
+ * know expands to about 30. */
+__device__ __forceinline__ int smallQuotient(int n, int d) {
+    return (int)(((float)n + .5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
 /* A ray's place in its pixel: the reference's stratified grid (lens.inc:367), or with a seed, a
  * jittered point in the same stratum. */
 __device__ __forceinline__ void subPixel(uint32_t seed, int px, int py, int sx, int sy, int side, float& dx, float& dy) {
@@ -30,7 +38,8 @@ __device__ __forceinline__ void subPixel(uint32_t seed, int px, int py, int sx, 
 
 /* The moment of sub-ray sub of side^2 within the shutter (lens.inc:371). */
 __device__ __forceinline__ float rayMoment(const ArtfHoleFrame& f, int sub, int side) {
-    const float moment = ((float)((sub * 5) % (side * side)) + .5f) * f.image[3] - .5f;
+    const int count = side * side, turned = sub * 5; /* below 320 */
+    const float moment = ((float)(turned - smallQuotient(turned, count) * count) + .5f) * f.image[3] - .5f;
     return f.disc[2] + f.spin[3] * moment;
 }
 
@@ -46,20 +55,44 @@ __device__ __forceinline__ void aimOf(const ArtfHoleFrame& f, int px, int py, fl
     ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)f.size[1]);
 }
 
+/* The sum and the largest of a 64-lane wave's values, every lane active: an inclusive scan by DPP moves
+ * within each row of 16 lanes (from 1, 2, 4 and 8 lanes below), then row 0's last lane added into row 1
+ * and row 2's into row 3, then row 1's last lane into rows 2 and 3, so that lane 63 holds the whole
+ * wave's; a lane whose source is outside its row, or a row the mask leaves out, takes the identity, 0.
+ * About a seventh of what the same reductions cost by cross-lane reads through LDS. */
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false); /* row_shr:1 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false); /* row_shr:2 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false); /* row_shr:4 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false); /* row_shr:8 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false); /* row_bcast:15 into rows 1, 3 */
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false); /* row_bcast:31 into rows 2, 3 */
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
+}
+
+__device__ __forceinline__ uint32_t waveMax(uint32_t v) {
+    int x = (int)v; /* values below 2^31: a signed max is the unsigned one */
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
+}
+
 /* The wave's step count and its lane-slots (its longest ray times its lanes with a pixel), added by
- * one lane to the counter slot of its workgroup (slots spread the atomics over many addresses). */
+ * one lane to the counter slot of its workgroup (slots spread the atomics over many addresses). Called
+ * by every lane of the wave. A ray takes at most 1,000,000 steps (the launch refuses more), so the
+ * wave's 64 fit 32 bits. */
 __device__ __forceinline__ void countSteps(unsigned long long* counters, int steps, bool valid) {
     if (!counters) return;
-    unsigned long long sum = (unsigned long long)steps;
-    int most = steps, lanes = valid ? 1 : 0;
-    for (int offset = 32; offset > 0; offset >>= 1) {
-        sum += __shfl_xor(sum, offset);
-        most = max(most, __shfl_xor(most, offset));
-        lanes += __shfl_xor(lanes, offset);
-    }
+    const uint32_t sum = waveSum((uint32_t)steps), most = waveMax((uint32_t)steps);
+    const uint32_t lanes = (uint32_t)__popcll(__ballot(valid));
     if ((threadIdx.x & 63) == 0 && lanes) {
         const unsigned slot = ((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x + (threadIdx.x >> 6)) % ARTF_HIP_COUNTER_SLOTS;
-        atomicAdd(&counters[slot], sum);
+        atomicAdd(&counters[slot], (unsigned long long)sum);
         atomicAdd(&counters[ARTF_HIP_COUNTER_SLOTS + slot], (unsigned long long)most * (unsigned long long)lanes);
     }
 }
@@ -82,10 +115,11 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wav
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
     const PixelTile tile = pixelTileOf(side);
     const int lane = (int)threadIdx.x;
-    const int pixel = lane / count, sub = lane - pixel * count; /* the lane's pixel in the tile, and its ray */
-    const int px = (int)blockIdx.x * tile.w + pixel % tile.w;
-    const int py = a.rowBegin + (int)blockIdx.y * tile.h + pixel / tile.w;
-    const int sx = sub % side, sy = sub / side;
+    const int pixel = smallQuotient(lane, count), sub = lane - pixel * count; /* the lane's pixel in the tile, and its ray */
+    const int pixelY = smallQuotient(pixel, tile.w), pixelX = pixel - pixelY * tile.w;
+    const int px = (int)blockIdx.x * tile.w + pixelX;
+    const int py = a.rowBegin + (int)blockIdx.y * tile.h + pixelY;
+    const int sy = smallQuotient(sub, side), sx = sub - sy * side;
     const bool valid = pixel < tile.w * tile.h && px < width && py < rowEnd;
 
     TraceLaw law;
@@ -118,24 +152,52 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wav
         }
     }
     countSteps(a.counters, ray.steps, valid);
-    float r = ray.r, g = ray.g, b = ray.b;
-    if (count > 1) {
-        /* the pixel's rays in its lanes: r = r + t ray by ray from 0 (sy, then sx), then times the share,
-         * in every lane of the pixel alike (count is the same for the whole wave) */
-        const int first = pixel * count;
-        float sr = 0.f, sg = 0.f, sb = 0.f;
-        for (int k = 0; k < count; ++k) {
-            sr = sr + __shfl(r, first + k);
-            sg = sg + __shfl(g, first + k);
-            sb = sb + __shfl(b, first + k);
-        }
-        r = sr * f.image[3], g = sg * f.image[3], b = sb * f.image[3];
+    if (count == 1) {
+        if (!valid) return;
+        const size_t at = (size_t)py * width + px;
+        a.out[0][at] = ray.r;
+        a.out[1][at] = ray.g;
+        a.out[2][at] = ray.b;
+        return;
     }
-    if (!valid || sub != 0) return;
-    const size_t at = (size_t)py * width + px;
-    a.out[0][at] = r;
-    a.out[1][at] = g;
-    a.out[2][at] = b;
+    /* The pixels' rays through LDS, then one lane a pixel and colour adds its pixel's rays in the order
+     * lensTile adds them, r = r + t ray by ray from 0 (sy, then sx), and scales by the share: 3 x pixels
+     * lanes each with a chain of `count` additions, where every lane walking all three colours' chains by
+     * cross-lane reads made the wave wait on each read. */
+    __shared__ __align__(16) float light[3][kTraceGroup];
+    light[0][lane] = ray.r;
+    light[1][lane] = ray.g;
+    light[2][lane] = ray.b;
+    __syncthreads();
+    const int pixels = tile.w * tile.h;
+    if (lane >= 3 * pixels) return;
+    const int colour = smallQuotient(lane, pixels), mine = lane - colour * pixels;
+    const int mineY = smallQuotient(mine, tile.w);
+    const int qx = (int)blockIdx.x * tile.w + mine - mineY * tile.w, qy = a.rowBegin + (int)blockIdx.y * tile.h + mineY;
+    if (qx >= width || qy >= rowEnd) return;
+    float* plane = colour == 0 ? a.out[0] : colour == 1 ? a.out[1] : a.out[2];
+    const float* rays = &light[colour][mine * count];
+    float sum = 0.f;
+    if ((count & 3) == 0) {
+        /* mine * count * 4 bytes from a 16-byte aligned array: whole float4s, four of them read before their
+         * sixteen additions where there are sixteen rays left */
+        const float4* quads = (const float4*)rays;
+        int k = 0;
+        for (; k + 4 <= count / 4; k += 4) {
+            const float4 t0 = quads[k], t1 = quads[k + 1], t2 = quads[k + 2], t3 = quads[k + 3];
+            sum = sum + t0.x, sum = sum + t0.y, sum = sum + t0.z, sum = sum + t0.w;
+            sum = sum + t1.x, sum = sum + t1.y, sum = sum + t1.z, sum = sum + t1.w;
+            sum = sum + t2.x, sum = sum + t2.y, sum = sum + t2.z, sum = sum + t2.w;
+            sum = sum + t3.x, sum = sum + t3.y, sum = sum + t3.z, sum = sum + t3.w;
+        }
+        for (; k < count / 4; ++k) {
+            const float4 t = quads[k];
+            sum = sum + t.x, sum = sum + t.y, sum = sum + t.z, sum = sum + t.w;
+        }
+    } else {
+        for (int k = 0; k < count; ++k) sum = sum + rays[k];
+    }
+    plane[(size_t)qy * width + qx] = sum * f.image[3];
 }
 
 inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHipTraceArgs& a, hipStream_t stream) {

@@ -267,7 +267,8 @@ def test_trace_every_side_equals_oracle(side, gpu, dev, native, oracle):
     """artfHipTrace at 1..8 rays a side (a wave holds 64 / side^2 pixels: 8 x 8, 4 x 4, 7 x 1, 2 x 2, 2 x 1
     and one pixel, with lanes left over at 3, 5, 6 and 7), on a frame that is no multiple of any of those
     tiles: each pixel's mean of its rays equals the oracle's, bit for bit, with a shutter (each ray its
-    own moment) and a moving eye; and the step counts are the oracle's."""
+    own moment) and a moving eye; and the step counts are the oracle's, and the lane-slots (each wave's
+    longest ray times its rays) those the oracle's steps give."""
     from tests.golden import cases
     case = dict(cases.CASES["moving_eye"])
     case["desc"] = dict(case["desc"], samples=side)
@@ -290,4 +291,12 @@ def test_trace_every_side_equals_oracle(side, gpu, dev, native, oracle):
     np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
     slots = dev.read(a.counters, 2 * SLOTS, np.uint64)
     assert int(slots[:SLOTS].sum()) == int(steps.astype(np.int64).sum())
+    # lane-slots: each wave's longest ray times its rays, a wave being tw x th pixels' rays
+    tw, th = {1: (8, 8), 2: (4, 4), 3: (7, 1), 4: (2, 2), 5: (2, 1)}.get(side, (1, 1))
+    want_slots = 0
+    for y in range(0, h, th):
+        for x in range(0, w, tw):
+            block = steps[:, y:y + th, x:x + tw].astype(np.int64)
+            want_slots += int(block.max()) * block.size
+    assert int(slots[SLOTS:].sum()) == want_slots
     s.close()
