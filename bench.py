@@ -560,6 +560,10 @@ def main():
     achieved = flops / (kernel["fp32"] * 1e-3) / 1e12
     traffic = traffic_from_profiles()
     cpu = None if args.no_cpu else cpu_baseline(native)
+    if cpu is not None and cpu["kind"] == "reference":
+        # the CPU frame traces the GPU frame's rays (its HDR is the GPU's bit for bit), so its steps are the
+        # frame's, which equal the oracle's count of the same rays (tests/golden/steps.json)
+        cpu["steps_per_ray"] = fp32["steps_per_frame"] / rays
     adjoint = adjoint_fit(native)
     log(f"adjoint: {adjoint}")
     renderer.close()
