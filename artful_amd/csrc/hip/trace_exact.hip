@@ -1,5 +1,6 @@
 /* artful-hip — the FP32 tracer that reproduces the reference's arithmetic: this file is built with
- * -ffp-contract=off, so no multiply-add is fused and each rounding is the reference's. */
+ * -ffp-contract=off, so the compiler fuses no multiply-add and each rounding is the reference's. The only
+ * fused ones are written out (scaledPlus, tracer.h): a product by 2 or 4, exact, added with one rounding. */
 #include "queue_kernel.h"
 
 namespace artf {

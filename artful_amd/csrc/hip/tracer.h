@@ -45,6 +45,13 @@ __device__ __forceinline__ double absOf(double x) { return fabs(x); }
  * (-fhip-fp32-correctly-rounded-divide-sqrt); never the native one-ulp instructions. */
 __device__ __forceinline__ float quot(float x, float y) { return x / y; }
 __device__ __forceinline__ double quot(double x, double y) { return x / y; }
+/* y + y + x, 4 y + x and the like with one rounding where the reference rounds twice: a product by 2 or 4
+ * is exact (short of overflow past 2^127, which no finite ray state comes near), so fusing it with the
+ * addition rounds once, at the same place the addition did, to the same bits. Zeros, denormals,
+ * infinities and NaNs come out alike too: the product's sign is the operand's, and the sum of two zeros
+ * follows the same rule in both. One instruction instead of two. */
+__device__ __forceinline__ float scaledPlus(float k, float y, float x) { return __builtin_fmaf(k, y, x); }
+__device__ __forceinline__ double scaledPlus(double k, double y, double x) { return __builtin_fma(k, y, x); }
 /* float to int by truncation, as cvttps2dq for every value the tracer forms. */
 template <typename R> __device__ __forceinline__ int32_t toInt(R x) { return (int32_t)x; }
 
@@ -213,7 +220,7 @@ __device__ __forceinline__ DiscTables tablesOf(const float* map, const float* fl
 
 template <typename R> __device__ __forceinline__ R bendOf(const RayConst<R>& c, R x) { return x * (c.uA + x * (c.uB + x * c.uC)); }
 template <typename R> __device__ __forceinline__ R dragOf(const RayConst<R>& c, R x) {
-    return quot(c.minusA * x * ((R)2 - c.xiA * x), hi((R)1 - (x + x) + c.a2 * x * x, (R)1e-3f));
+    return quot(c.minusA * x * ((R)2 - c.xiA * x), hi(scaledPlus((R)-2, x, (R)1) + c.a2 * x * x, (R)1e-3f)); /* 1 - (x + x) */
 }
 
 /* The ray through a pixel, and its constants (lens.inc:101-163). */
@@ -290,18 +297,19 @@ __device__ __forceinline__ void nystromStep(const RayConst<R>& c, const RayState
     const R k3 = bendOf(c, endU);
     const R c3 = big + twoA2 * (e2 * e2);
     const R g1x = zero - c3 * e0, g1y = zero - c3 * e1, g1z = a2 * e2 - c3 * e2;
-    n.u = s.u + step * s.w + sixthSquare * (k1 + (k2 + k2));
-    n.w = s.w + sixth * ((k1 + k3) + (R)4 * k2);
-    n.n0 = s.n0 + step * s.v0 + sixthSquare * (a1x + (b1x + b1x));
-    n.n1 = s.n1 + step * s.v1 + sixthSquare * (a1y + (b1y + b1y));
-    n.n2 = s.n2 + step * s.v2 + sixthSquare * (a1z + (b1z + b1z));
-    n.v0 = s.v0 + sixth * ((a1x + g1x) + (R)4 * b1x);
-    n.v1 = s.v1 + sixth * ((a1y + g1y) + (R)4 * b1y);
-    n.v2 = s.v2 + sixth * ((a1z + g1z) + (R)4 * b1z);
+    /* k1 + (k2 + k2), (k1 + k3) + 4 k2 and the like, each with its exact doubling or quadrupling fused */
+    n.u = s.u + step * s.w + sixthSquare * scaledPlus((R)2, k2, k1);
+    n.w = s.w + sixth * scaledPlus((R)4, k2, k1 + k3);
+    n.n0 = s.n0 + step * s.v0 + sixthSquare * scaledPlus((R)2, b1x, a1x);
+    n.n1 = s.n1 + step * s.v1 + sixthSquare * scaledPlus((R)2, b1y, a1y);
+    n.n2 = s.n2 + step * s.v2 + sixthSquare * scaledPlus((R)2, b1z, a1z);
+    n.v0 = s.v0 + sixth * scaledPlus((R)4, b1x, a1x + g1x);
+    n.v1 = s.v1 + sixth * scaledPlus((R)4, b1y, a1y + g1y);
+    n.v2 = s.v2 + sixth * scaledPlus((R)4, b1z, a1z + g1z);
     n.drag = zero, n.psi = zero;
     if (c.spinning) {
         n.drag = dragOf(c, n.u);
-        n.psi = s.psi + sixth * ((s.drag + n.drag) + (R)4 * dragOf(c, midU));
+        n.psi = s.psi + sixth * scaledPlus((R)4, dragOf(c, midU), s.drag + n.drag);
     }
     looks.k1 = k1, looks.a1x = a1x, looks.a1y = a1y, looks.a1z = a1z;
 }
