@@ -1,6 +1,7 @@
 /* artful-hip — the FP32 tracer that reproduces the reference's arithmetic: this file is built with
  * -ffp-contract=off, so the compiler fuses no multiply-add and each rounding is the reference's. The only
- * fused ones are written out (scaledPlus, tracer.h): a product by 2 or 4, exact, added with one rounding. */
+ * fused ones are written out (scaledPlus, tracer.h): an exact product by a power of two added with one rounding,
+ * where the reference's two roundings give the same bits. */
 #include "queue_kernel.h"
 
 namespace artf {

@@ -45,11 +45,13 @@ __device__ __forceinline__ double absOf(double x) { return fabs(x); }
  * (-fhip-fp32-correctly-rounded-divide-sqrt); never the native one-ulp instructions. */
 __device__ __forceinline__ float quot(float x, float y) { return x / y; }
 __device__ __forceinline__ double quot(double x, double y) { return x / y; }
-/* y + y + x, 4 y + x and the like with one rounding where the reference rounds twice: a product by 2 or 4
- * is exact (short of overflow past 2^127, which no finite ray state comes near), so fusing it with the
- * addition rounds once, at the same place the addition did, to the same bits. Zeros, denormals,
- * infinities and NaNs come out alike too: the product's sign is the operand's, and the sum of two zeros
- * follows the same rule in both. One instruction instead of two. */
+/* y + y + x, 4 y + x and the like with one rounding where the reference rounds twice. A product by 2 or 4
+ * is exact short of overflow past 2^127, which no finite ray state comes near, so fusing it with the
+ * addition rounds once, at the same place the addition did, to the same bits; zeros, denormals,
+ * infinities and NaNs come out alike too (the product's sign is the operand's, and the sum of two zeros
+ * follows one rule in both). A product by 1/2 or 1/4 is exact but in the denormal range, and where it is
+ * used here (.5 m + .5, .25 warm + .75) a denormal-sized term lies far below the last bit of the sum, which
+ * is then the addend either way. One instruction instead of two. */
 __device__ __forceinline__ float scaledPlus(float k, float y, float x) { return __builtin_fmaf(k, y, x); }
 __device__ __forceinline__ double scaledPlus(double k, double y, double x) { return __builtin_fma(k, y, x); }
 /* float to int by truncation, as cvttps2dq for every value the tracer forms. */
@@ -106,7 +108,7 @@ __device__ __forceinline__ void starLayer(R dx_, R dy_, R dz_, float cells, int 
     R light = hi((R)1 - d2 * (R)(1.f / (size * size)), (R)0);
     light = light * light * (R)gain;
     const R warm = hashByte<R>(h, 3);
-    r = r + light * ((R).75f + warm * (R).25f);
+    r = r + light * scaledPlus((R).25f, warm, (R).75f); /* .75 + warm .25 */
     g = g + light * (R).82f;
     b = b + light * ((R)1 - warm * (R).45f);
 }
@@ -166,7 +168,7 @@ struct RayResult {
 /* smoothstep on [0, 1] for the analytic disc's soft edges. */
 template <typename R> __device__ __forceinline__ R edgeRamp(R t) {
     t = lo(hi(t, (R)0), (R)1);
-    return t * t * ((R)3 - (R)2 * t);
+    return t * t * scaledPlus((R)-2, t, (R)3); /* 3 - 2 t */
 }
 
 /* x^q for x > 0, through exp2/log2 in the ray's precision. */
@@ -459,7 +461,7 @@ __device__ __forceinline__ void skyLight(float stars, const RayConst<R>& c, R aw
     starLayer(sx, sy, m, 90.f, 26, .1f, 1.6f, r, g, b);
     starLayer(sx, sy, m, 240.f, 70, .17f, .22f, r, g, b);
     const R skyTurn = turnsOf(sy, sx);
-    const R heightAt = (m * (R).5f + (R).5f) * (R)(float)(ARTF_HOLE_SKY_HEIGHT - 1);
+    const R heightAt = scaledPlus((R).5f, m, (R).5f) * (R)(float)(ARTF_HOLE_SKY_HEIGHT - 1); /* m .5 + .5 */
     const R skyFloor = lo(floorOf(heightAt), (R)(float)(ARTF_HOLE_SKY_HEIGHT - 2));
     const int32_t skyRow = toInt(skyFloor);
     const R s0 = rowAt(map + ARTF_HOLE_SKY_AT, ARTF_HOLE_SKY_WIDTH, skyRow, skyTurn);
