@@ -123,6 +123,23 @@ template <typename R> __device__ __forceinline__ R rowAt(const float* __restrict
     return a + (b - a) * part;
 }
 
+/* table[i] for i below 2^30, as a 32-bit byte offset from the table's base: the load takes it as it is. */
+__device__ __forceinline__ float texel(const float* __restrict__ table, uint32_t i) {
+    return *(const float*)((const char*)table + (i << 2));
+}
+
+/* rowAt for the disc's tables (gas, fluid, dust): their rows are never negative (rho in (0, 1) picks them,
+ * where the sky's can fall to -1), so the offsets are unsigned 32-bit ones from the table's base, which the
+ * loads take as they are (no 64-bit sign extension and add per load). The same texels, the same light. */
+template <typename R> __device__ __forceinline__ R discRowAt(const float* __restrict__ table, uint32_t width, uint32_t row, R turn) {
+    const R at = (turn - floorOf(turn)) * (R)(float)width;
+    const R column = floorOf(at), part = at - column;
+    const uint32_t base = row * width;
+    const uint32_t c0 = (uint32_t)toInt(column) & (width - 1u), c1 = (c0 + 1u) & (width - 1u);
+    const R a = (R)texel(table, base + c0), b = (R)texel(table, base + c1);
+    return a + (b - a) * part;
+}
+
 /* What a ray needs of the frame, in its own precision. */
 template <typename R> struct Eye {
     R forward[3], right[3], up[3];
@@ -366,23 +383,23 @@ __device__ __forceinline__ void crossPainted(const DiscConst<R>& d, const RayCon
         const R rowAtF = rhoK * (R)(float)(ARTF_HOLE_MAP_HEIGHT - 1);
         const R rowFloor = lo(floorOf(rowAtF), (R)(float)(ARTF_HOLE_MAP_HEIGHT - 2));
         const R rowPart = rowAtF - rowFloor;
-        const int32_t row0 = toInt(rowFloor), row1 = row0 + 1;
-        const R speed0 = (R)rows[row0 + row0], speed1 = (R)rows[row1 + row1];
+        const uint32_t row0 = (uint32_t)toInt(rowFloor), row1 = row0 + 1u; /* rhoK in (0, 1): 0..1023 */
+        const R speed0 = (R)texel(rows, row0 + row0), speed1 = (R)texel(rows, row1 + row1);
         const R turnK = turn + rate * ds;
-        const R d0 = rowAt(map, ARTF_HOLE_MAP_WIDTH, row0, turnK - speed0 * d.time);
-        const R d1 = rowAt(map, ARTF_HOLE_MAP_WIDTH, row1, turnK - speed1 * d.time);
+        const R d0 = discRowAt(map, ARTF_HOLE_MAP_WIDTH, row0, turnK - speed0 * d.time);
+        const R d1 = discRowAt(map, ARTF_HOLE_MAP_WIDTH, row1, turnK - speed1 * d.time);
         R density = d0 + (d1 - d0) * rowPart;
         /* the fluid as last stepped, each ring turned on since */
         const R fluidAt = rhoK * (R)(float)(ARTF_FLUID_HEIGHT - 1);
         const R fluidFloor = lo(floorOf(fluidAt), (R)(float)(ARTF_FLUID_HEIGHT - 2));
-        const int32_t fluidRow = toInt(fluidFloor);
+        const uint32_t fluidRow = (uint32_t)toInt(fluidFloor);
         const R fluidTurn = turnK - speed0 * d.flowLag;
-        const R q0 = rowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow, fluidTurn), q1 = rowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow + 1, fluidTurn);
+        const R q0 = discRowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow, fluidTurn), q1 = discRowAt(tables.fluid, ARTF_FLUID_WIDTH, fluidRow + 1u, fluidTurn);
         const R gas = q0 + (q1 - q0) * (fluidAt - fluidFloor);
         density = density * ((R)(1.f - ARTF_HOLE_FLUID) + (R)ARTF_HOLE_FLUID * gas);
         const R dustAt = rhoK * (R)(float)(ARTF_HOLE_DUST_HEIGHT - 1);
-        const int32_t dustRow = toInt(lo(floorOf(dustAt), (R)(float)(ARTF_HOLE_DUST_HEIGHT - 1)));
-        const R dusty = rowAt(map + ARTF_HOLE_DUST_AT, ARTF_HOLE_DUST_WIDTH, dustRow, turnK - speed0 * d.time);
+        const uint32_t dustRow = (uint32_t)toInt(lo(floorOf(dustAt), (R)(float)(ARTF_HOLE_DUST_HEIGHT - 1)));
+        const R dusty = discRowAt(map + ARTF_HOLE_DUST_AT, ARTF_HOLE_DUST_WIDTH, dustRow, turnK - speed0 * d.time);
         const R lit = one - (R)ARTF_HOLE_DUST_DIM * dusty;
         const R height = t * quot(reach, d.discHeight);
         const R level = hi(one - vz * vz * (height * height), zero);
@@ -394,9 +411,9 @@ __device__ __forceinline__ void crossPainted(const DiscConst<R>& d, const RayCon
     }
     /* temperature, and what the gas's orbit and the hole's pull do to it */
     const R rhoSafe = lo(hi(rhoCenter, zero), one);
-    const int32_t rowIndex = toInt(rhoSafe * (R)(float)(ARTF_HOLE_MAP_HEIGHT - 1));
-    R heat = (R)rows[rowIndex + rowIndex + 1];
-    const R gives = (R)tables.glow[rowIndex];
+    const uint32_t rowIndex = (uint32_t)toInt(rhoSafe * (R)(float)(ARTF_HOLE_MAP_HEIGHT - 1)); /* rhoSafe in [0, 1] */
+    R heat = (R)texel(rows, rowIndex + rowIndex + 1u);
+    const R gives = (R)texel(tables.glow, rowIndex);
     R light = d.gain * glow * gives;
     if (d.doppler != zero) {
         const R g = orbitShift(c, uc, d.doppler);
