@@ -23,6 +23,12 @@ struct Planes {
  * drain, and the next frame's trace would then wait for it. */
 constexpr int kPost = 64;
 
+/* The post kernels of a frame run beside the next frame's trace, whose waves (96 VGPRs, 5 a SIMD) leave 32
+ * VGPRs a SIMD: a wave that needs more waits for the trace to stop dispatching, and the frame's chain of
+ * post launches behind it. So these take at most 32 (gfx950 reads this attribute as half the registers of
+ * its unified file: 16 here is 32). */
+#define ARTF_POST_REGISTERS __attribute__((amdgpu_num_vgpr(16)))
+
 /* Mean of a pixel's side^2 rays, summed in the order lensTile adds them, times 1/side^2. */
 __global__ __launch_bounds__(kPost) void resolveKernel(Planes rays, Planes hdr, int pixels, int count, float share) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,7 +81,7 @@ __device__ void settleCounters(const Settle& z) {
 
 /* Level 0: the mean of 2 x 2 pixels, each first held to what exceeds the threshold, at most the
  * ceiling (software.c:336-349). */
-__global__ __launch_bounds__(kPost) void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp,
+__global__ __launch_bounds__(kPost) ARTF_POST_REGISTERS void bloomFirstKernel(Planes hdr, Planes bloom, Level to, int width, int height, unsigned long long* stamp,
                                                         Settle z) {
     stampStart(stamp);
     settleCounters(z);
@@ -111,14 +117,14 @@ __device__ __forceinline__ float upGain(const float* level, const Level& from, i
     return sum * (.25f * ARTF_BLOOM_FADE);
 }
 
-__global__ __launch_bounds__(kPost) void bloomDownKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+__global__ __launch_bounds__(kPost) ARTF_POST_REGISTERS void bloomDownKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
     stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
     for (int c = 0; c < 3; ++c) bloom.p[c][to.at + (size_t)y * to.w + x] = downTexel(bloom.p[c] + from.at, from, x, y);
 }
 
-__global__ __launch_bounds__(kPost) void bloomUpKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
+__global__ __launch_bounds__(kPost) ARTF_POST_REGISTERS void bloomUpKernel(Planes bloom, Level from, Level to, unsigned long long* stamp) {
     stampStart(stamp);
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= (int)to.w) return;
@@ -231,7 +237,7 @@ __device__ __forceinline__ void tonePixel(const Planes& hdr, const Planes& bloom
 /* kToneRun pixels of a row a lane, kPost apart (coalesced): fewer, longer workgroups. */
 constexpr int kToneRun = 4;
 
-__global__ __launch_bounds__(kPost) void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
+__global__ __launch_bounds__(kPost) ARTF_POST_REGISTERS void toneKernel(Planes hdr, Planes bloom, uint32_t* out, Level l0, int width, float exposure, float spread,
                                                   ToneStamps stamps) {
     stampStart(stamps.start, stamps.alsoStart);
     const int y = (int)blockIdx.y;
@@ -272,9 +278,10 @@ static int smallFirst(const ArtfHoleFrame& f) {
 
 /* The pyramid above level 0, which is in place (software.c:419-426): levels 1..s down one launch each,
  * the small end in one workgroup, and levels s..0 back up one launch each (s the small end's first
- * level). stamp: given to the first launch; returns whether there was one. */
-static bool bloomRest(const ArtfHoleFrame& f, const Planes& b, hipStream_t stream, unsigned long long* stamp) {
-    const int last = (int)f.size[3] - 1, small = smallFirst(f);
+ * level; without smallEnd, every level one launch each way). stamp: given to the first launch; returns
+ * whether there was one. */
+static bool bloomRest(const ArtfHoleFrame& f, const Planes& b, hipStream_t stream, unsigned long long* stamp, bool smallEnd) {
+    const int last = (int)f.size[3] - 1, small = smallEnd ? smallFirst(f) : last;
     bool any = false;
     auto first = [&]() {
         unsigned long long* at = any ? nullptr : stamp;
@@ -304,7 +311,7 @@ hipError_t launchBloom(const ArtfHoleFrame& f, float* const hdr[3], float* const
     const Level l0 = levelOf(f, 0);
     hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, stream, h, b, l0, (int)f.size[0], (int)f.size[1], (unsigned long long*)nullptr,
                        Settle{});
-    bloomRest(f, b, stream, nullptr);
+    bloomRest(f, b, stream, nullptr, true);
     return hipGetLastError();
 }
 
@@ -337,7 +344,9 @@ hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
             return e;
     }
     hipLaunchKernelGGL(bloomFirstKernel, rowsOf(l0), dim3(kPost), 0, post, h, b, l0, (int)f.size[0], (int)f.size[1], r ? r + 3 : nullptr, z);
-    const bool rest = bloomRest(f, b, post, r ? r + 4 : nullptr);
+    /* beside a trace (a second stream) the small end goes level by level: its one-wave kernel needs more
+     * registers than a SIMD running the trace has left; alone, one launch instead of 2 (levels - s) - 1 */
+    const bool rest = bloomRest(f, b, post, r ? r + 4 : nullptr, post == stream);
     const ToneStamps ts = r ? ToneStamps{r + 5, rest ? nullptr : r + 4, r + 6} : ToneStamps{};
     hipLaunchKernelGGL(toneKernel, pixelRows(f), dim3(kPost), 0, post, h, b, a.bgra, l0, (int)f.size[0], f.image[0], spreadOf(f), ts);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -108,7 +108,7 @@ __host__ __device__ constexpr PixelTile pixelTileOf(int side) {
 }
 
 template <int Precision>
-__global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
+__global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
     stampStart(a.startStamp, a.spanStamp);
     const ArtfHoleFrame& f = a.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2], count = side * side;
