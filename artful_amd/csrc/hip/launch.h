@@ -18,10 +18,13 @@ constexpr int kTraceGroup = 64;
  * measures 5% faster (4 spills too much). */
 constexpr int wavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP64 ? 3 : 4; }
 
-/* The tile kernel: the same, but 5 for the exact FP32 tracer, which fits 96 VGPRs without spilling. More
+/* The tile kernel: the same, but 5 for the FP32 tracers, which fit 96 VGPRs (the exact one with one value
+ * spilled, stored as a ray starts and read as it ends; 6 waves would spill 12 and run 3% slower). More
  * waves to choose from raise the SIMDs' two-instructions-a-cycle issue (the kernel alone 2% faster); in a
  * frame, the post passes' waves (at most 32 VGPRs each) still fit in the 32 a SIMD then has left. */
-constexpr int traceWavesPerSimd(int precision) { return precision == ARTF_PRECISION_FP32 ? 5 : wavesPerSimd(precision); }
+constexpr int traceWavesPerSimd(int precision) {
+    return precision == ARTF_PRECISION_FP32 || precision == ARTF_PRECISION_FP32_FAST ? 5 : wavesPerSimd(precision);
+}
 
 /* The GPU's clock (wall_clock64: artfHipWallClockHz ticks a second) stored to *at by the launch's first
  * lane of its first workgroup as it starts, with a system-scope store (at may be host memory): a stage

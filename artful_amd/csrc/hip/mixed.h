@@ -72,15 +72,15 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
             else crossAnalytic(d, cf, s, n, looks, law.q, map, acc);
         }
         const bool escaped = uEscape > n.u;
-        if (escaped) {
-            const float part = quot(s.u - uEscape, s.u - n.u);
-            awayX = s.n0 + part * (n.n0 - s.n0);
-            awayY = s.n1 + part * (n.n1 - s.n1);
-            awayZ = s.n2 + part * (n.n2 - s.n2);
-            awayPsi = s.psi + part * (n.psi - s.psi);
-            sky = true;
-        }
-        if (escaped || n.u > uHorizon || !(acc.through > .004f)) {
+        if (escaped || n.u > uHorizon || !(acc.through > .004f)) { /* where it left, only as it leaves: as traceRay */
+            if (escaped) {
+                const float part = quot(s.u - uEscape, s.u - n.u);
+                awayX = s.n0 + part * (n.n0 - s.n0);
+                awayY = s.n1 + part * (n.n1 - s.n1);
+                awayZ = s.n2 + part * (n.n2 - s.n2);
+                awayPsi = s.psi + part * (n.psi - s.psi);
+                sky = true;
+            }
             steps = i + 1;
             break;
         }

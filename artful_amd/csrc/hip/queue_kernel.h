@@ -107,15 +107,17 @@ template <typename R> struct PlainLane {
         }
         const R uEscape = (R)w.uEscape;
         const bool escaped = uEscape > n.u;
-        if (escaped) {
-            const R part = quot(s.u - uEscape, s.u - n.u);
-            away.x = s.n0 + part * (n.n0 - s.n0);
-            away.y = s.n1 + part * (n.n1 - s.n1);
-            away.z = s.n2 + part * (n.n2 - s.n2);
-            away.psi = s.psi + part * (n.psi - s.psi);
-            sky = true;
+        if (escaped || n.u > uHorizonOf(v) || !(acc.through > (R).004f)) { /* where it left, only as it leaves: as traceRay */
+            if (escaped) {
+                const R part = quot(s.u - uEscape, s.u - n.u);
+                away.x = s.n0 + part * (n.n0 - s.n0);
+                away.y = s.n1 + part * (n.n1 - s.n1);
+                away.z = s.n2 + part * (n.n2 - s.n2);
+                away.psi = s.psi + part * (n.psi - s.psi);
+                sky = true;
+            }
+            return true;
         }
-        if (escaped || n.u > uHorizonOf(v) || !(acc.through > (R).004f)) return true;
         s = n;
         return false;
     }
@@ -176,15 +178,17 @@ struct MixedLane {
             else crossAnalytic(d, cf, s, n, looks, v.emissivity, w.tables.map, acc);
         }
         const bool escaped = uEscape > n.u;
-        if (escaped) {
-            const float part = quot(s.u - uEscape, s.u - n.u);
-            away.x = s.n0 + part * (n.n0 - s.n0);
-            away.y = s.n1 + part * (n.n1 - s.n1);
-            away.z = s.n2 + part * (n.n2 - s.n2);
-            away.psi = s.psi + part * (n.psi - s.psi);
-            sky = true;
+        if (escaped || n.u > uHorizon || !(acc.through > .004f)) { /* where it left, only as it leaves: as traceRay */
+            if (escaped) {
+                const float part = quot(s.u - uEscape, s.u - n.u);
+                away.x = s.n0 + part * (n.n0 - s.n0);
+                away.y = s.n1 + part * (n.n1 - s.n1);
+                away.z = s.n2 + part * (n.n2 - s.n2);
+                away.psi = s.psi + part * (n.psi - s.psi);
+                sky = true;
+            }
+            return true;
         }
-        if (escaped || n.u > uHorizon || !(acc.through > .004f)) return true;
         s = n;
         return false;
     }

@@ -527,9 +527,9 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
     R awayX = 0, awayY = 0, awayZ = 0, awayPsi = 0;
     const R uHorizon = e.uHorizon, uEscape = (R)law.uEscape;
     int steps = law.maxSteps;
-    RayState<R> n = s;
 #pragma unroll 2 /* two steps a trip round the loop: fewer branches and moves, 2.4% (5% with FMA) */
     for (int i = 0; i < law.maxSteps; ++i) {
+        RayState<R> n;
         StepLooks<R> looks;
         nystromStep(c, s, n, looks);
         if (s.n2 * n.n2 < (R)0) { /* through the disc's plane */
@@ -537,21 +537,23 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
             else crossAnalytic(d, c, s, n, looks, law.q, map, acc);
         }
         const bool escaped = uEscape > n.u;
+        /* A lane leaves the loop at escape, at the horizon or opaque. Where an escaping ray left is worked
+         * out here, where only waves with a lane leaving go: as a branch of its own every step it cost every
+         * wave its exec-mask work at every step, and after the loop it kept both states alive across it
+         * (18 more register moves a step). */
         if (escaped || n.u > uHorizon || !(acc.through > (R).004f)) {
-            sky = escaped;
+            if (escaped) {
+                const R part = quot(s.u - uEscape, s.u - n.u);
+                awayX = s.n0 + part * (n.n0 - s.n0);
+                awayY = s.n1 + part * (n.n1 - s.n1);
+                awayZ = s.n2 + part * (n.n2 - s.n2);
+                awayPsi = s.psi + part * (n.psi - s.psi);
+                sky = true;
+            }
             steps = i + 1;
             break;
         }
         s = n;
-    }
-    /* where an escaping ray left: between its last two states, at u = uEscape (after the loop, where the
-     * per-step branch for it cost every wave its exec-mask work at every step) */
-    if (sky) {
-        const R part = quot(s.u - uEscape, s.u - n.u);
-        awayX = s.n0 + part * (n.n0 - s.n0);
-        awayY = s.n1 + part * (n.n1 - s.n1);
-        awayZ = s.n2 + part * (n.n2 - s.n2);
-        awayPsi = s.psi + part * (n.psi - s.psi);
     }
     if (f.look[2] > 0.f && sky) skyLight(f.look[2], c, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
     RayResult out;
