@@ -1,14 +1,18 @@
 /* artful-hip — the adjoint kernel: each lane runs one ray forward and back (adjoint.h), and each
  * workgroup (one wave over an 8 x 8 tile, as the tile tracer) sums its rays' gradients into 32 doubles
  * (a fixed order: the sums are the same every run). Built without contraction, so its forward light
- * equals trace_exact.hip's analytic-disc light. */
+ * equals trace_exact.hip's analytic-disc light (the reverse pass's own arithmetic fuses: adjoint.h). */
 #include "adjoint.h"
 #include "trace_kernel.h"
 
 namespace artf {
 
+/* Waves a SIMD the adjoint's registers are held to: the FP32 one at 3 (168 VGPRs; its next-step loads
+ * ahead take 175 and 2 waves otherwise, 3% slower); the FP64 one keeps what it needs (1 wave). */
+constexpr int adjointWavesPerSimd(int bytes) { return bytes == 4 ? 3 : 1; }
+
 template <typename R>
-__global__ __launch_bounds__(kTraceGroup) void adjointKernel(ArtfHipAdjointArgs args) {
+__global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(adjointWavesPerSimd(sizeof(R))))) void adjointKernel(ArtfHipAdjointArgs args) {
     const ArtfHipTraceArgs& a = args.trace;
     const ArtfHoleFrame& f = a.frame;
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
@@ -40,7 +44,7 @@ __global__ __launch_bounds__(kTraceGroup) void adjointKernel(ArtfHipAdjointArgs 
             const double ny = 1.0 - ((double)py + (double)dy) * (2.0 / (double)height);
             const double seed[3] = {(double)args.seed[0][pixel] * f.image[3], (double)args.seed[1][pixel] * f.image[3],
                                     (double)args.seed[2][pixel] * f.image[3]};
-            Saved<double> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT];
+            Saved<double> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT + 1];
             double g[ADJ_COUNT];
             for (int k = 0; k < ADJ_COUNT; ++k) g[k] = 0.0;
             adjointRay<double>(f, e, law, a.map, nx, ny, time, seed, light, g, ckpt, seg);
@@ -51,7 +55,7 @@ __global__ __launch_bounds__(kTraceGroup) void adjointKernel(ArtfHipAdjointArgs 
             const float nx = ((float)(px & 7) + ((float)(px & ~7) + dx)) * f.image[1] - 1.f;
             const float ny = 1.f - ((float)py + dy) * f.image[2];
             const float seed[3] = {args.seed[0][pixel] * f.image[3], args.seed[1][pixel] * f.image[3], args.seed[2][pixel] * f.image[3]};
-            Saved<float> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT];
+            Saved<float> ckpt[ARTF_ADJ_SLOTS], seg[ARTF_ADJ_CHECKPOINT + 1];
             float g[ADJ_COUNT];
             for (int k = 0; k < ADJ_COUNT; ++k) g[k] = 0.f;
             adjointRay<float>(f, e, law, a.map, nx, ny, time, seed, light, g, ckpt, seg);

@@ -218,14 +218,16 @@ def adjoint_fit(native, iterations=200):
     history = K.fit(ctx, target, iterations=iterations)
     wall = time.perf_counter() - t0
     ctx.close()
-    best = min(range(len(history)), key=lambda k: history[k][0])
+    least, params = K.best(history)
     return {"forward_ms": float(np.median(fwd)), "adjoint_ms": float(np.median(adj)),
             "adjoint_over_forward": float(np.median(adj) / np.median(fwd)),
             "fit": {"iterations": iterations, "wall_s": wall, "loss_first": history[0][0], "loss_last": history[-1][0],
-                    "loss_best": history[best][0], "params_last": [float(v) for v in history[-1][1]],
+                    "loss_best": least, "params_best": [float(v) for v in params], "params_last": [float(v) for v in history[-1][1]],
                     "hidden": list(K.HIDDEN), "start": list(K.START),
-                    "params_error": {n: float(v - h) for n, v, h in zip(K.PARAMS, history[-1][1], K.HIDDEN)},
-                    "preconditioner": "Gauss-Newton whitening from forward central differences, refreshed every 25 iterations"}}
+                    "params_error": {n: float(v - h) for n, v, h in zip(K.PARAMS, params, K.HIDDEN)},
+                    "params_error_last": {n: float(v - h) for n, v, h in zip(K.PARAMS, history[-1][1], K.HIDDEN)},
+                    "result": "the iterate of least loss",
+                    "preconditioner": "Gauss-Newton whitening from forward central differences, refreshed every 25 iterations; iterates held within bounds"}}
 
 
 def per_view_tiles(native, views, trc, reps=3):

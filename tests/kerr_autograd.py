@@ -85,6 +85,19 @@ def gauss_newton(renderer, params, steps=FD_STEPS) -> np.ndarray:
     return 2.0 / J.shape[0] * (J.T @ J)
 
 
+# where the fit's parameters are held (spin, inclination in degrees, inner, outer, q); outer stays at
+# least OUTER_GAP beyond inner. Past a spin of .998 the scene clamps and the image stops changing, so an
+# iterate that left these would find no gradient back.
+BOUNDS = np.array([[0.0, .99], [1.0, 89.0], [2.0, 10.0], [8.0, 60.0], [.5, 6.0]])
+OUTER_GAP = 2.0
+
+
+def within_bounds(params: np.ndarray) -> np.ndarray:
+    p = np.clip(params, BOUNDS[:, 0], BOUNDS[:, 1])
+    p[3] = max(p[3], p[2] + OUTER_GAP)
+    return p
+
+
 def fit(renderer, target: np.ndarray, iterations=200, start=START, lr=.01, refresh=25, floor=1e-3):
     """Adam on the five parameters, from start toward the image target, on the adjoint's gradients; the
     loss and parameters at each iteration.
@@ -93,7 +106,15 @@ def fit(renderer, target: np.ndarray, iterations=200, start=START, lr=.01, refre
     size per parameter stalls in the valley between them), so Adam runs in whitened coordinates: every
     `refresh` iterations the Gauss-Newton matrix H = V diag(lam) V^T is taken at the current point
     (eigenvalues held above `floor` x the largest), and p = p_ref + V diag(lam^-1/2) z, z optimised from 0
-    by a fresh Adam whose learning rate follows a cosine from lr down to lr / 20 over the whole fit."""
+    by a fresh Adam whose learning rate follows a cosine from lr down to lr / 20 over the whole fit. An
+    iterate outside BOUNDS is moved back onto them (z set to the bounded point's coordinates).
+
+    Along the valley the loss is rippled (the integrator's discrete steps move against the disc as the
+    parameters change): its local slope, the adjoint's and a finite difference's alike, is ten times the
+    valley's overall slope, so where along the valley the fit passes closest to the target and where it
+    stops depend on the start and on the gradients' last bits. Its result is the iterate of least loss
+    (`best`), which lies nearer the target than the last one: each refresh's fresh Adam first moves
+    every coordinate by the learning rate, and near the target that step costs about what is left."""
     t = torch.from_numpy(np.asarray(target, np.float64))
     p = np.array(start, np.float64)
     history = []
@@ -102,6 +123,7 @@ def fit(renderer, target: np.ndarray, iterations=200, start=START, lr=.01, refre
         lam, V = np.linalg.eigh(gauss_newton(renderer, p))
         lam = np.maximum(lam, lam.max() * floor)
         to_params = torch.from_numpy(V @ np.diag(lam ** -.5))
+        from_params = torch.from_numpy(np.diag(lam ** .5) @ V.T)
         p_ref = torch.from_numpy(p.copy())
         z = torch.zeros(5, dtype=torch.float64, requires_grad=True)
         opt = torch.optim.Adam([z], lr=lr)
@@ -114,6 +136,16 @@ def fit(renderer, target: np.ndarray, iterations=200, start=START, lr=.01, refre
             loss.backward()
             history.append((float(loss.detach()), params.detach().numpy().copy()))
             opt.step()
+            with torch.no_grad():
+                moved = (p_ref + to_params @ z).numpy()
+                held = within_bounds(moved)
+                if np.any(held != moved):
+                    z.copy_(from_params @ torch.from_numpy(held - p_ref.numpy()))
             k += 1
         p = (p_ref + to_params @ z).detach().numpy()
     return history
+
+
+def best(history):
+    """The fit's result: the (loss, parameters) of its iterate of least loss."""
+    return min(history, key=lambda h: h[0])
