@@ -4,17 +4,19 @@
  * FP64 is kept for
  *   - the ray's setup: direction, aberration, the constants of motion (energy, xi, eta) and K, from
  *     the frame worked out in doubles; the FP32 coefficients are these, rounded once;
- *   - the radial and polar turning points: a step in which u' (w) or m' (v.z) changes sign is taken
- *     again in FP64 from the FP32 state, widened;
- *   - the step next to the horizon: a step whose FP32 result has u past its capture value is taken
- *     again in FP64, which decides whether the ray is captured there. (FP64 for every step with u
- *     past .9 of the capture value as well costs 6% more and changes one image of the 4K study in 16,
- *     by 2% of its mean error: a captured ray's light is what it gathered before.)
- * The FP32 state after such a step is the FP64 result, rounded. Everything else, the disc-plane
- * crossings, the escape and the light they give included, is the FP32 arithmetic of tracer.h (as
- * the FP32-with-FMA tracer does it). Taking the crossings and the escape in FP64 as well gave a third
- * of this mean error against the FP64 tracer, but no more speed than the FP64 tracer itself. Not in
- * the reference (which is FP32 only). */
+ *   - the radial turning point and the horizon: a step in which u' (w) changes sign, or whose FP32 u is
+ *     past its capture value, has its radial half (u, u') taken again in FP64 from the FP32 state,
+ *     widened; FP64 then decides whether the ray is captured there;
+ *   - the polar turning point: a step in which m' (v.z) changes sign has its polar axis (m = n.z and m')
+ *     taken again in FP64.
+ * In Mino time u'' depends on u alone, and m'' = a^2 m - (K + 2 a^2 m^2) m on m alone, so what is redone
+ * is the FP64 step's own result for those variables. The other two components of n (the azimuth) and
+ * the drag angle psi, a quadrature of u, stay FP32, as does everything else: the disc-plane crossings,
+ * the escape and the light they give are the FP32 arithmetic of tracer.h (as the FP32-with-FMA tracer
+ * does it). Redoing the whole step in FP64 at each of these (drag included) gave the same error against
+ * the FP64 tracer at 1080p (mean 2.33e-5 against 2.39e-5 now) and took 7% longer; taking the crossings and
+ * the escape in FP64 as well gave a third of this mean error, but no more speed than the FP64 tracer
+ * itself. Not in the reference (which is FP32 only). */
 #pragma once
 
 #include "tracer.h"
@@ -38,6 +40,25 @@ __device__ __forceinline__ RayConst<float> narrowConst(const RayConst<double>& c
     return t;
 }
 
+/* One mixed step: the FP32 step, then in FP64 again the radial half (u, u') where u' changes sign or u
+ * passes the capture value, and the polar axis (m, m') where m' changes sign; the FP32 state takes the
+ * FP64 results, rounded. */
+__device__ __forceinline__ void mixedStep(const RayConst<double>& cd, const RayConst<float>& cf, const RayState<float>& s, float uHorizon,
+                                          RayState<float>& n, StepLooks<float>& looks) {
+    nystromStep(cf, s, n, looks);
+    if (s.w * n.w <= 0.f || n.u > uHorizon) {
+        double u, w, k1, midU;
+        radialStep(cd, (double)s.u, (double)s.w, u, w, k1, midU);
+        n.u = (float)u, n.w = (float)w, looks.k1 = (float)k1;
+    }
+    if (s.v2 * n.v2 <= 0.f) {
+        double m, v, a1z;
+        PolarFactors<double> k;
+        polarAxisStep(cd, (double)s.n2, (double)s.v2, m, v, a1z, k);
+        n.n2 = (float)m, n.v2 = (float)v, looks.a1z = (float)a1z;
+    }
+}
+
 __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const ArtfHoleFrame64& f64, const TraceLaw& law, const float* __restrict__ map,
                                    const float* __restrict__ fluid, double nx, double ny, float time) {
     Eye<double> e;
@@ -58,15 +79,7 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
     for (int i = 0; i < law.maxSteps; ++i) {
         RayState<float> n;
         StepLooks<float> looks;
-        nystromStep(cf, s, n, looks);
-        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uHorizon) { /* this step again, in FP64 */
-            const RayState<double> w = castState<double>(s);
-            RayState<double> m;
-            StepLooks<double> l;
-            nystromStep(cd, w, m, l);
-            n = castState<float>(m);
-            looks.k1 = (float)l.k1, looks.a1x = (float)l.a1x, looks.a1y = (float)l.a1y, looks.a1z = (float)l.a1z;
-        }
+        mixedStep(cd, cf, s, uHorizon, n, looks);
         if (s.n2 * n.n2 < 0.f) {
             if (law.disc == DISC_PAINTED) crossPainted(d, cf, s, n, looks, tablesOf(map, fluid), acc);
             else crossAnalytic(d, cf, s, n, looks, law.q, map, acc);

@@ -155,22 +155,14 @@ struct MixedLane {
         acc.r = acc.g = acc.b = 0.f, acc.through = 1.f;
     }
 
-    /* traceRayMixed's step: FP64 again at turning points and for the step that reaches the horizon, the
-     * rest in FP32 */
+    /* traceRayMixed's step (mixedStep: FP32, with the radial half or the polar axis again in FP64 at
+     * turning points and for the step that reaches the horizon) */
     __device__ __forceinline__ bool step(const WaveView& w, float time, bool& sky) {
         const ArtfHipView& v = *w.v;
         const float uHorizon = (float)v.frame64.spin[1], uEscape = w.uEscape;
         RayState<float> n;
         StepLooks<float> looks;
-        nystromStep(cf, s, n, looks);
-        if (s.w * n.w <= 0.f || s.v2 * n.v2 <= 0.f || n.u > uHorizon) {
-            const RayState<double> x = castState<double>(s);
-            RayState<double> m;
-            StepLooks<double> l;
-            nystromStep(cd, x, m, l);
-            n = castState<float>(m);
-            looks.k1 = (float)l.k1, looks.a1x = (float)l.a1x, looks.a1y = (float)l.a1y, looks.a1z = (float)l.a1z;
-        }
+        mixedStep(cd, cf, s, uHorizon, n, looks);
         if (s.n2 * n.n2 < 0.f) {
             DiscConst<float> d;
             discOfView(v, time, d);

@@ -296,41 +296,84 @@ template <typename R> __device__ __forceinline__ void setupRay(const Eye<R>& e, 
     s.drag = c.spinning ? dragOf(c, s.u) : zero;
 }
 
-/* One Nystrom step (lens.inc:167-192). */
+/* The radial half of a Nystrom step (lens.inc:167-192): u and u' = w, which depend on u alone; also the
+ * first stage's u'' (k1) and the middle stage's u (midU), which the crossing and the drag read. */
 template <typename R>
-__device__ __forceinline__ void nystromStep(const RayConst<R>& c, const RayState<R>& s, RayState<R>& n, StepLooks<R>& looks) {
-    const R zero = 0;
+__device__ __forceinline__ void radialStep(const RayConst<R>& c, R u, R w, R& nu, R& nw, R& k1, R& midU) {
+    const R step = c.step, step2 = c.step2;
+    const R half = step * (R).5f, eighth = step2 * (R).125f, halfSquare = step2 * (R).5f;
+    const R sixthSquare = step2 * (R)(1.f / 6.f), sixth = step * (R)(1.f / 6.f);
+    k1 = bendOf(c, u);
+    midU = u + half * w + eighth * k1;
+    const R k2 = bendOf(c, midU);
+    const R endU = u + step * w + halfSquare * k2;
+    const R k3 = bendOf(c, endU);
+    /* k1 + (k2 + k2) and (k1 + k3) + 4 k2, each with its exact doubling or quadrupling fused */
+    nu = u + step * w + sixthSquare * scaledPlus((R)2, k2, k1);
+    nw = w + sixth * scaledPlus((R)4, k2, k1 + k3);
+}
+
+/* The three stages' factors K + 2 a^2 m^2 of the polar equations. */
+template <typename R> struct PolarFactors {
+    R c1, c2, c3;
+};
+
+/* The polar half's axis component: m = n.z and m' = v.z, whose equation m'' = a^2 m - (K + 2 a^2 m^2) m
+ * involves m alone; also its first stage's m'' and the stages' factors, which the other two components
+ * take. */
+template <typename R>
+__device__ __forceinline__ void polarAxisStep(const RayConst<R>& c, R m, R v, R& nm, R& nv, R& a1z, PolarFactors<R>& k) {
     const R step = c.step, step2 = c.step2, big = c.big, twoA2 = c.twoA2, a2 = c.a2;
     const R half = step * (R).5f, eighth = step2 * (R).125f, halfSquare = step2 * (R).5f;
     const R sixthSquare = step2 * (R)(1.f / 6.f), sixth = step * (R)(1.f / 6.f);
-    const R k1 = bendOf(c, s.u);
-    const R c1 = big + twoA2 * (s.n2 * s.n2);
-    const R a1x = zero - c1 * s.n0, a1y = zero - c1 * s.n1, a1z = a2 * s.n2 - c1 * s.n2;
-    const R midU = s.u + half * s.w + eighth * k1;
-    const R m0 = s.n0 + half * s.v0 + eighth * a1x, m1 = s.n1 + half * s.v1 + eighth * a1y, m2 = s.n2 + half * s.v2 + eighth * a1z;
-    const R k2 = bendOf(c, midU);
-    const R c2 = big + twoA2 * (m2 * m2);
-    const R b1x = zero - c2 * m0, b1y = zero - c2 * m1, b1z = a2 * m2 - c2 * m2;
-    const R endU = s.u + step * s.w + halfSquare * k2;
-    const R e0 = s.n0 + step * s.v0 + halfSquare * b1x, e1 = s.n1 + step * s.v1 + halfSquare * b1y, e2 = s.n2 + step * s.v2 + halfSquare * b1z;
-    const R k3 = bendOf(c, endU);
-    const R c3 = big + twoA2 * (e2 * e2);
-    const R g1x = zero - c3 * e0, g1y = zero - c3 * e1, g1z = a2 * e2 - c3 * e2;
-    /* k1 + (k2 + k2), (k1 + k3) + 4 k2 and the like, each with its exact doubling or quadrupling fused */
-    n.u = s.u + step * s.w + sixthSquare * scaledPlus((R)2, k2, k1);
-    n.w = s.w + sixth * scaledPlus((R)4, k2, k1 + k3);
+    k.c1 = big + twoA2 * (m * m);
+    a1z = a2 * m - k.c1 * m;
+    const R m2 = m + half * v + eighth * a1z;
+    k.c2 = big + twoA2 * (m2 * m2);
+    const R b1z = a2 * m2 - k.c2 * m2;
+    const R e2 = m + step * v + halfSquare * b1z;
+    k.c3 = big + twoA2 * (e2 * e2);
+    const R g1z = a2 * e2 - k.c3 * e2;
+    /* a + (b + b) and (a + g) + 4 b, each with its exact doubling or quadrupling fused */
+    nm = m + step * v + sixthSquare * scaledPlus((R)2, b1z, a1z);
+    nv = v + sixth * scaledPlus((R)4, b1z, a1z + g1z);
+}
+
+/* The polar half: n on the unit sphere and n', which depend on n alone; also the first stage's n''. */
+template <typename R>
+__device__ __forceinline__ void polarStep(const RayConst<R>& c, const RayState<R>& s, RayState<R>& n, StepLooks<R>& looks) {
+    const R zero = 0;
+    const R step = c.step, step2 = c.step2;
+    const R half = step * (R).5f, eighth = step2 * (R).125f, halfSquare = step2 * (R).5f;
+    const R sixthSquare = step2 * (R)(1.f / 6.f), sixth = step * (R)(1.f / 6.f);
+    PolarFactors<R> k;
+    polarAxisStep(c, s.n2, s.v2, n.n2, n.v2, looks.a1z, k);
+    const R a1x = zero - k.c1 * s.n0, a1y = zero - k.c1 * s.n1;
+    const R m0 = s.n0 + half * s.v0 + eighth * a1x, m1 = s.n1 + half * s.v1 + eighth * a1y;
+    const R b1x = zero - k.c2 * m0, b1y = zero - k.c2 * m1;
+    const R e0 = s.n0 + step * s.v0 + halfSquare * b1x, e1 = s.n1 + step * s.v1 + halfSquare * b1y;
+    const R g1x = zero - k.c3 * e0, g1y = zero - k.c3 * e1;
     n.n0 = s.n0 + step * s.v0 + sixthSquare * scaledPlus((R)2, b1x, a1x);
     n.n1 = s.n1 + step * s.v1 + sixthSquare * scaledPlus((R)2, b1y, a1y);
-    n.n2 = s.n2 + step * s.v2 + sixthSquare * scaledPlus((R)2, b1z, a1z);
     n.v0 = s.v0 + sixth * scaledPlus((R)4, b1x, a1x + g1x);
     n.v1 = s.v1 + sixth * scaledPlus((R)4, b1y, a1y + g1y);
-    n.v2 = s.v2 + sixth * scaledPlus((R)4, b1z, a1z + g1z);
+    looks.a1x = a1x, looks.a1y = a1y;
+}
+
+/* One Nystrom step (lens.inc:167-192): the radial and polar halves (the equations for u and for n do not
+ * touch each other in Mino time), then the drag angle psi by Simpson's rule, from u alone. */
+template <typename R>
+__device__ __forceinline__ void nystromStep(const RayConst<R>& c, const RayState<R>& s, RayState<R>& n, StepLooks<R>& looks) {
+    const R zero = 0;
+    R midU;
+    radialStep(c, s.u, s.w, n.u, n.w, looks.k1, midU);
+    polarStep(c, s, n, looks);
     n.drag = zero, n.psi = zero;
     if (c.spinning) {
+        const R sixth = c.step * (R)(1.f / 6.f);
         n.drag = dragOf(c, n.u);
         n.psi = s.psi + sixth * scaledPlus((R)4, dragOf(c, midU), s.drag + n.drag);
     }
-    looks.k1 = k1, looks.a1x = a1x, looks.a1y = a1y, looks.a1z = a1z;
 }
 
 /* The Doppler and gravitational shift g of gas on a circular orbit at uc, for this ray
