@@ -37,6 +37,7 @@ __device__ __forceinline__ RayConst<float> narrowConst(const RayConst<double>& c
     t.uA = (float)c.uA, t.uB = (float)c.uB, t.uC = (float)c.uC, t.xiA = (float)c.xiA;
     t.xi = (float)c.xi, t.invEnergy = (float)c.invEnergy, t.moved = (float)c.moved;
     t.spinning = c.spinning;
+    t.inRangeDrag = c.inRangeDrag;
     return t;
 }
 
@@ -66,7 +67,8 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
     RayConst<double> cd;
     RayState<double> sd;
     setupRay(e, nx, ny, cd, sd);
-    const RayConst<float> cf = narrowConst(cd);
+    RayConst<float> cf = narrowConst(cd);
+    cf.inRangeDrag = law.inRangeDrag;
     RayState<float> s = castState<float>(sd);
     DiscConst<float> d;
     discOf<float>(f, f.disc[0], f.disc[1], f.disc[3], f.spin[2], time, d);

@@ -45,6 +45,21 @@ __device__ __forceinline__ double absOf(double x) { return fabs(x); }
  * (-fhip-fp32-correctly-rounded-divide-sqrt); never the native one-ulp instructions. */
 __device__ __forceinline__ float quot(float x, float y) { return x / y; }
 __device__ __forceinline__ double quot(double x, double y) { return x / y; }
+/* x / y correctly rounded by the steps of the IEEE sequence hipcc emits for it (the reciprocal refined once,
+ * the quotient corrected twice), without its scaling (v_div_scale, twice) and fix-up (v_div_fixup): those
+ * change nothing where y, 1/y and x/y are normal numbers and x is neither tiny nor zero, and here the sign
+ * of a zero quotient is put back as IEEE gives it for y > 0 (x's). So, for 2^-20 <= y <= 2^20 and either
+ * 2^-100 <= |x| <= 2^60, or x = 0, or y = 1 (the steps then return x itself, denormals included), the
+ * quotient is x / y's, bit for bit, in 9 instructions instead of 11. Checked on the CPU with the hardware
+ * reciprocal taken one ulp either way (tests/test_kernel_arith.py). */
+__device__ __forceinline__ float quotInRange(float x, float y) {
+    const float r0 = __builtin_amdgcn_rcpf(y);
+    const float r = __builtin_fmaf(__builtin_fmaf(-y, r0, 1.f), r0, r0);
+    const float q0 = x * r;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-y, q0, x), r, q0);
+    return __builtin_copysignf(__builtin_fmaf(__builtin_fmaf(-y, q1, x), r, q1), x);
+}
+__device__ __forceinline__ double quotInRange(double x, double y) { return x / y; }
 /* y + y + x, 4 y + x and the like with one rounding where the reference rounds twice. A product by 2 or 4
  * is exact short of overflow past 2^127, which no finite ray state comes near, so fusing it with the
  * addition rounds once, at the same place the addition did, to the same bits; zeros, denormals,
@@ -175,7 +190,12 @@ struct TraceLaw {
     float uEscape;  /* the reference: 0 */
     int disc;       /* DISC_PAINTED or DISC_ANALYTIC */
     float q;        /* the analytic disc's emissivity index */
+    bool inRangeDrag; /* the steps' drag by quotInRange (RayConst::inRangeDrag); a kernel's constant */
 };
+
+/* Whether a frame's spin lets the steps take the drag's quotient by quotInRange (dragOf): 0 (no drag) or at
+ * least 2^-50 in size. */
+__host__ __device__ inline bool dragInRange(float spin) { return spin == 0.f || spin >= 0x1p-50f || spin <= -0x1p-50f; }
 
 struct RayResult {
     float r, g, b;
@@ -199,6 +219,7 @@ template <typename R> struct RayConst {
     R uA, uB, uC, xiA;   /* u'' = u (uA + u (uB + u uC)); xi a */
     R xi, invEnergy, moved;
     bool spinning;
+    bool inRangeDrag; /* the drag's quotient by quotInRange (the spin is 0 or at least 2^-50: dragOf) */
 };
 
 /* Where a ray is: u = 1/r and u', n on the unit sphere and n', the drag angle psi and its rate. */
@@ -238,8 +259,16 @@ __device__ __forceinline__ DiscTables tablesOf(const float* map, const float* fl
 }
 
 template <typename R> __device__ __forceinline__ R bendOf(const RayConst<R>& c, R x) { return x * (c.uA + x * (c.uB + x * c.uC)); }
+/* The drag's rate -a x (2 - xi a x) / max(1 - 2x + a^2 x^2, 1e-3) at u = x. Its divisor lies in [1e-3, 2^20]
+ * for |x| <= 2^9, and a stage point's u stays within a few units (the loop ends once u passes the capture
+ * value, 1/r_h <= 1, and a step moves u by a fraction of itself). Where |x| < 2^-26 the divisor is 1 exactly
+ * (1 - 2x rounds to 1 and a^2 x^2 is far below its last bit); elsewhere the dividend is 0 or at least |a| 2^-49
+ * in size (2 - xi a x is 0 or a multiple of 2^-23, since xi a x near 2 leaves an exact difference, and
+ * |a x| >= |a| 2^-26), so at least 2^-99 for a spin of at least 2^-50. quotInRange then gives the IEEE
+ * quotient: c.inRangeDrag, set where the frame's spin is 0 or at least 2^-50 (dragInRange). */
 template <typename R> __device__ __forceinline__ R dragOf(const RayConst<R>& c, R x) {
-    return quot(c.minusA * x * ((R)2 - c.xiA * x), hi(scaledPlus((R)-2, x, (R)1) + c.a2 * x * x, (R)1e-3f)); /* 1 - (x + x) */
+    const R num = c.minusA * x * ((R)2 - c.xiA * x), den = hi(scaledPlus((R)-2, x, (R)1) + c.a2 * x * x, (R)1e-3f); /* 1 - (x + x) */
+    return c.inRangeDrag ? quotInRange(num, den) : quot(num, den);
 }
 
 /* The ray through a pixel, and its constants (lens.inc:101-163). */
@@ -285,6 +314,7 @@ template <typename R> __device__ __forceinline__ void setupRay(const Eye<R>& e, 
     c.xiA = xi * a;
     c.xi = xi, c.invEnergy = invEnergy, c.moved = moved;
     c.spinning = a != zero;
+    c.inRangeDrag = false; /* the caller's to set, for the steps */
 
     s.u = u0;
     s.w = ((-u0 * u0 * rootDelta * rho) * dx) * invEnergy;
@@ -563,6 +593,7 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
     RayConst<R> c;
     RayState<R> s;
     setupRay(e, nx, ny, c, s);
+    c.inRangeDrag = law.inRangeDrag;
     DiscConst<R> d;
     discFrom(f, e, time, d);
     Gathered<R> acc = {(R)0, (R)0, (R)0, (R)1};

@@ -127,6 +127,21 @@ def test_extended_law_fp32_bit_exact(renderer, native, oracle, scene_of):
     np.testing.assert_array_equal(hdr.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("spin", [1e-20, -3e-16, 2.0 ** -50, -(2.0 ** -50) * 1.5, 1e-6, .3])
+def test_spin_either_side_of_the_drag_bound(renderer, native, oracle, spin):
+    """The steps take the drag's quotient without the division's scaling when the frame's spin is 0 or at
+    least 2^-50 in size (dragInRange), by IEEE division below that: the oracle's bits either way."""
+    desc, cam = cases.make(cases.CASES["config1"])
+    desc.spin = spin
+    s = native.Scene(desc, cam)
+    f, _ = s.frame(96, 54)
+    assert f.spin[0] == np.float32(spin)
+    want = oracle.trace(f, s.map(), s.fluid()[0], 32)
+    hdr, _ = draw(renderer, s, 96, 54)
+    np.testing.assert_array_equal(hdr.view(np.uint32), want.view(np.uint32))
+    s.close()
+
+
 def test_1080p_bands_bit_exact(gpu, native, oracle):
     """The 1080p workload (2x2 rays a pixel, spin .998) against the oracle on bands of rows, and the
     frame's size-independent properties: finite, non-negative, the same twice."""
