@@ -116,7 +116,8 @@ class HipBatchArgs(C.Structure):
     _fields_ = [("views", C.c_void_p), ("firstTile", C.c_void_p), ("viewCount", C.c_int32), ("tileBegin", C.c_uint32),
                 ("tileEnd", C.c_uint32),
                 ("precision", C.c_int32), ("map", C.c_void_p), ("rings", C.c_void_p), ("fluids", C.c_void_p),
-                ("queue", C.c_void_p), ("counters", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule)]
+                ("queue", C.c_void_p), ("counters", C.c_void_p), ("stamps", C.c_void_p), ("schedule", HipSchedule),
+                ("dragInRange", C.c_int32)]
 
 
 class HoleView(C.Structure):

@@ -86,10 +86,11 @@ template <typename R> struct PlainLane {
     Gathered<R> acc;
     Away<R> away;
 
-    __device__ __forceinline__ void begin(const WaveView& w, R nx, R ny) {
+    __device__ __forceinline__ void begin(const WaveView& w, R nx, R ny, bool inRangeDrag) {
         Eye<R> e;
         eyeOfView<R>(*w.v, e);
         setupRay(e, nx, ny, c, s);
+        c.inRangeDrag = inRangeDrag;
         acc.r = acc.g = acc.b = (R)0, acc.through = (R)1;
     }
 
@@ -145,12 +146,13 @@ struct MixedLane {
     Gathered<float> acc;
     Away<float> away;
 
-    __device__ __forceinline__ void begin(const WaveView& w, double nx, double ny) {
+    __device__ __forceinline__ void begin(const WaveView& w, double nx, double ny, bool inRangeDrag) {
         Eye<double> e;
         eyeFrom64(w.v->frame64, e);
         RayState<double> sd;
         setupRay(e, nx, ny, cd, sd);
         cf = narrowConst(cd);
+        cf.inRangeDrag = inRangeDrag;
         s = castState<float>(sd);
         acc.r = acc.g = acc.b = 0.f, acc.through = 1.f;
     }
@@ -225,7 +227,7 @@ __device__ __forceinline__ void stampView(const ArtfHipBatchArgs& a, uint32_t vi
 
 /* views and firstTile come apart from the rest as restrict arguments: nothing the kernel writes can
  * alias them, so their wave-uniform reads are scalar loads. */
-template <int Precision>
+template <int Precision, bool InRangeDrag>
 __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wavesPerSimd(Precision)))) void queueKernel(const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile,
                                                            ArtfHipBatchArgs a) {
     using Lane = typename LaneOf<Precision>::type;
@@ -324,7 +326,7 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
                     time = rayMoment(f, (int)sub, side);
                     typename Lane::Aim nx, ny;
                     aimOf(f, px, py, dx, dy, nx, ny);
-                    ray.begin(w, nx, ny);
+                    ray.begin(w, nx, ny, InRangeDrag);
                     outAt = (side == 1 ? 0u : sub * (uint32_t)width * (uint32_t)height) + (uint32_t)py * (uint32_t)width + (uint32_t)px;
                     steps = 0, sky = false, active = true;
                 }
@@ -360,15 +362,21 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
     }
 }
 
+/* The batch's kernel, with the drag's quotient in range where every view's spin allows it. */
 template <int Precision> inline hipError_t launchQueueWith(const ArtfHipBatchArgs& a, hipStream_t stream) {
-    hipLaunchKernelGGL(queueKernel<Precision>, dim3((unsigned)a.schedule.workgroups), dim3(kQueueGroup), 0, stream, a.views, a.firstTile, a);
+    void (*kernel)(const ArtfHipView*, const uint32_t*, ArtfHipBatchArgs) =
+        a.dragInRange ? queueKernel<Precision, true> : queueKernel<Precision, false>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)a.schedule.workgroups), dim3(kQueueGroup), 0, stream, a.views, a.firstTile, a);
     return hipGetLastError();
 }
 
+/* Workgroups of the batch's kernel a compute unit holds (the fewer of its two forms). */
 template <int Precision> inline int queueOccupancy() {
-    int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, queueKernel<Precision>, kQueueGroup, 0) != hipSuccess) return 0;
-    return blocks;
+    int blocks = 0, other = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, queueKernel<Precision, false>, kQueueGroup, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&other, queueKernel<Precision, true>, kQueueGroup, 0) != hipSuccess)
+        return 0;
+    return blocks < other ? blocks : other;
 }
 
 } // namespace artf

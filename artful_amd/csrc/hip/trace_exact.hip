@@ -10,7 +10,7 @@ template __global__ void traceKernel<ARTF_PRECISION_FP32, false>(ArtfHipTraceArg
 template __global__ void traceKernel<ARTF_PRECISION_FP32, true>(ArtfHipTraceArgs);
 
 hipError_t launchTraceExact(const ArtfHipTraceArgs& a, hipStream_t stream) {
-    return launchTraceWith(dragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_FP32, true> : traceKernel<ARTF_PRECISION_FP32, false>, a, stream);
+    return launchTraceWith(artfHipDragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_FP32, true> : traceKernel<ARTF_PRECISION_FP32, false>, a, stream);
 }
 
 hipError_t launchViewsExact(const ArtfHipBatchArgs& a, hipStream_t stream) { return launchQueueWith<ARTF_PRECISION_FP32>(a, stream); }

@@ -8,9 +8,9 @@ namespace artf {
 hipError_t launchTraceFast(const ArtfHipTraceArgs& a, hipStream_t stream) {
     switch (a.precision) {
     case ARTF_PRECISION_FP32_FAST:
-        return launchTraceWith(dragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_FP32_FAST, true> : traceKernel<ARTF_PRECISION_FP32_FAST, false>, a, stream);
+        return launchTraceWith(artfHipDragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_FP32_FAST, true> : traceKernel<ARTF_PRECISION_FP32_FAST, false>, a, stream);
     case ARTF_PRECISION_MIXED:
-        return launchTraceWith(dragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_MIXED, true> : traceKernel<ARTF_PRECISION_MIXED, false>, a, stream);
+        return launchTraceWith(artfHipDragInRange(a.frame.spin[0]) ? traceKernel<ARTF_PRECISION_MIXED, true> : traceKernel<ARTF_PRECISION_MIXED, false>, a, stream);
     case ARTF_PRECISION_FP64: return launchTrace64(a, stream);
     default: return hipErrorInvalidValue;
     }

@@ -108,7 +108,7 @@ __host__ __device__ constexpr PixelTile pixelTileOf(int side) {
 }
 
 /* InRangeDrag: the FP32 steps take the drag's quotient by quotInRange, for a frame whose spin allows it
- * (dragInRange); the same bits as with IEEE division, two instructions fewer for each of a step's two. */
+ * (artfHipDragInRange); the same bits as with IEEE division, two instructions fewer for each of a step's two. */
 template <int Precision, bool InRangeDrag = false>
 __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
     stampStart(a.startStamp, a.spanStamp);

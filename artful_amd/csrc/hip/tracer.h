@@ -193,10 +193,6 @@ struct TraceLaw {
     bool inRangeDrag; /* the steps' drag by quotInRange (RayConst::inRangeDrag); a kernel's constant */
 };
 
-/* Whether a frame's spin lets the steps take the drag's quotient by quotInRange (dragOf): 0 (no drag) or at
- * least 2^-50 in size. */
-__host__ __device__ inline bool dragInRange(float spin) { return spin == 0.f || spin >= 0x1p-50f || spin <= -0x1p-50f; }
-
 struct RayResult {
     float r, g, b;
     int steps;
@@ -265,7 +261,7 @@ template <typename R> __device__ __forceinline__ R bendOf(const RayConst<R>& c, 
  * (1 - 2x rounds to 1 and a^2 x^2 is far below its last bit); elsewhere the dividend is 0 or at least |a| 2^-49
  * in size (2 - xi a x is 0 or a multiple of 2^-23, since xi a x near 2 leaves an exact difference, and
  * |a x| >= |a| 2^-26), so at least 2^-99 for a spin of at least 2^-50. quotInRange then gives the IEEE
- * quotient: c.inRangeDrag, set where the frame's spin is 0 or at least 2^-50 (dragInRange). */
+ * quotient: c.inRangeDrag, set where the frame's spin is 0 or at least 2^-50 (artfHipDragInRange). */
 template <typename R> __device__ __forceinline__ R dragOf(const RayConst<R>& c, R x) {
     const R num = c.minusA * x * ((R)2 - c.xiA * x), den = hi(scaledPlus((R)-2, x, (R)1) + c.a2 * x * x, (R)1e-3f); /* 1 - (x + x) */
     return c.inRangeDrag ? quotInRange(num, den) : quot(num, den);

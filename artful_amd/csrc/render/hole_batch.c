@@ -207,6 +207,8 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     a->views = b->deviceViews, a->firstTile = b->firstTile, a->viewCount = count, a->tileBegin = 0, a->tileEnd = at;
     b->tiles = at;
     a->precision = t.precision, a->map = b->map, a->rings = b->rings, a->fluids = b->fluids;
+    a->dragInRange = 1;
+    for (int v = 0; v < count; ++v) a->dragInRange &= artfHipDragInRange(b->views[v].frame.spin[0]);
     b->queues[0] = (uint32_t*)w, b->queues[1] = (uint32_t*)(w + queueBytes);
     a->queue = b->queues[0];
     a->counters = (unsigned long long*)(w + 2 * queueBytes);

@@ -10,6 +10,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "artful/core.h"
 #include "artful/hole.h"
 
 #ifdef __cplusplus
@@ -128,6 +129,12 @@ typedef struct ArtfHipView {
 #define ARTF_HIP_QUEUES 8
 #define ARTF_HIP_QUEUE_STRIDE 32
 
+/* Whether a frame's spin lets the FP32 steps take the drag's quotient -a u (2 - xi a u) / max(1 - 2u + a^2 u^2,
+ * 1e-3) by the IEEE division's own steps without its scaling and fix-up (the same quotient, two instructions
+ * fewer): a spin of 0 (no drag) or of at least 2^-50 in size. artfHipTrace and the frames choose by it
+ * themselves; a batch says so in ArtfHipBatchArgs.dragInRange. */
+ARTF_INLINE int artfHipDragInRange(float spin) { return spin == 0.f || spin >= 0x1p-50f || spin <= -0x1p-50f; }
+
 /* How the persistent launch spends the device. */
 typedef struct ArtfHipSchedule {
     int32_t workgroups;    /* of 256 lanes (four independent waves); at least 1 */
@@ -165,6 +172,10 @@ typedef struct ArtfHipBatchArgs {
      * (artfHipWallClockHz). Spread over slots, as the counters are. */
     unsigned long long* stamps;
     ArtfHipSchedule schedule;
+    /* 1 only if every view's frame.spin[0] passes artfHipDragInRange: the FP32 steps then take the drag's
+     * quotient without the division's scaling (the same bits, fewer instructions); 0: by IEEE division,
+     * right for any spin */
+    int32_t dragInRange;
 } ArtfHipBatchArgs;
 
 /* Clears the queue, the counters and the stamps, and stamps the start, with a kernel (no

@@ -70,6 +70,19 @@ def test_fp32_painted_equals_renderer(gpu, renderer):
     assert 0 < stats.steps <= stats.laneSlots
 
 
+def test_fp32_tiny_spin_view_equals_oracle(gpu, oracle):
+    """A batch with one view whose spin is below 2^-50 in size takes every view's drag by IEEE division
+    (ArtfHipBatchArgs.dragInRange 0), the others by the in-range steps: the oracle's bits either way."""
+    views = [view_of(1e-20, 60, 40, 20, 48, 27), view_of(.9, 70, 50, 20, 40, 24)]
+    images, _ = batch_images(gpu, views, precision="fp32", disc="painted")
+    for k, (desc, cam, w, h) in enumerate(views):
+        s = gpu.Scene(desc, cam)
+        f, _ = s.frame(w, h)
+        want = oracle.trace(f, s.map(), s.fluid()[0], 32)
+        assert same_bits(images[k], want), f"view {k}"
+        s.close()
+
+
 @pytest.mark.parametrize("precision", ["fp32_fast", "mixed", "fp64"])
 def test_other_precisions_match_renderer(gpu, renderer, precision):
     tracing = dict(precision=precision, disc="painted")
