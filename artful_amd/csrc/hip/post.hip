@@ -5,6 +5,7 @@
  * reference's, operation for operation (src/software/software.c:320-386; lens.inc:364-380, 387-440),
  * so the bytes of a frame equal the reference's for the same light. */
 #include "launch.h"
+#include "trace_kernel.h"
 #include "tracer.h"
 
 namespace artf {
@@ -325,12 +326,37 @@ hipError_t launchToneMap(const ArtfHoleFrame& f, float* const hdr[3], float* con
     return hipGetLastError();
 }
 
+/* A frame of more than ARTF_HIP_RAYS_APART rays is traced in bands of whole rows of tiles, a launch each, one
+ * after another on the trace's stream, each of at least ARTF_HIP_RAYS_A_BAND rays: as the reference's Vulkan
+ * backend sends such a frame out (vulkan.c:1729-1755), so that no launch holds the GPU for long (here ~6 ms
+ * a band, against 46 ms for a 3840x2160 frame of 64 rays a pixel in one launch; each band's last waves
+ * drain alone, 1% of such a frame in 9 bands, 2% in 17). Rays do not depend on each other, so the image is
+ * the same bits; the first band stamps the trace's start. */
+static hipError_t launchTraceBands(ArtfHipTraceArgs t, hipStream_t stream) {
+    const int width = (int)t.frame.size[0], height = (int)t.frame.size[1], side = (int)t.frame.size[2];
+    const unsigned long long perRow = (unsigned long long)width * (unsigned long long)(side * side);
+    if (perRow * (unsigned long long)height <= ARTF_HIP_RAYS_APART || side < 1 || side > 8)
+        return t.precision == ARTF_PRECISION_FP32 ? launchTraceExact(t, stream) : launchTraceFast(t, stream);
+    const int tileRows = pixelTileOf(side).h;
+    const unsigned long long perTileRow = perRow * (unsigned long long)tileRows;
+    const int band = (int)((ARTF_HIP_RAYS_A_BAND + perTileRow - 1) / perTileRow) * tileRows;
+    for (int y = 0; y < height; y += band) {
+        ArtfHipTraceArgs b = t;
+        b.rowBegin = y, b.rowEnd = min(y + band, height);
+        if (y > 0) b.startStamp = b.spanStamp = nullptr;
+        const hipError_t e = b.precision == ARTF_PRECISION_FP32 ? launchTraceExact(b, stream) : launchTraceFast(b, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launchFrame(const ArtfHipFrameArgs& a, hipStream_t stream) {
     const ArtfHoleFrame& f = a.trace.frame;
     unsigned long long* r = a.report;
     ArtfHipTraceArgs t = a.trace;
     t.startStamp = r ? r + 2 : nullptr;
-    const hipError_t traced = t.precision == ARTF_PRECISION_FP32 ? launchTraceExact(t, stream) : launchTraceFast(t, stream);
+    t.rowBegin = 0, t.rowEnd = 0; /* a frame is traced whole */
+    const hipError_t traced = launchTraceBands(t, stream);
     if (traced != hipSuccess) return traced;
     Planes h = {{a.hdr[0], a.hdr[1], a.hdr[2]}}, b = {{a.bloom[0], a.bloom[1], a.bloom[2]}};
     const Level l0 = levelOf(f, 0);

@@ -289,6 +289,12 @@ typedef struct ArtfHipFrameArgs {
     void* done;
 } ArtfHipFrameArgs;
 
+/* Frames of more than ARTF_HIP_RAYS_APART rays are traced in bands of rows of at least ARTF_HIP_RAYS_A_BAND
+ * rays, a launch each, one after another on the call's stream (the reference's Vulkan backend splits a frame
+ * of more than 64,000,000 rays into bands too, src/vulkan/vulkan.c:1729-1755); the same image. */
+#define ARTF_HIP_RAYS_APART 64000000ull
+#define ARTF_HIP_RAYS_A_BAND 64000000ull
+
 int artfHipDrawFrame(const ArtfHipFrameArgs* args, void* stream);
 
 /* artfHipDrawFrame for callers that cannot allocate device memory (code under HIP graph capture, tight
