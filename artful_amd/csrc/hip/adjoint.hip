@@ -27,6 +27,7 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(adj
     law.uEscape = a.escapeRadius > 0.f ? 1.f / a.escapeRadius : 0.f;
     law.disc = DISC_ANALYTIC;
     law.q = a.emissivity;
+    law.inRangeDrag = false; /* the adjoint's rays take no drag (adjointRay) */
 
     double grad[ADJ_COUNT];
     for (int k = 0; k < ADJ_COUNT; ++k) grad[k] = 0.0;
