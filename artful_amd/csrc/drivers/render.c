@@ -9,15 +9,18 @@
  *                 [--roll degrees] [--fov degrees] [--exposure x] [--bloom x] [--around degrees]
  *                 [--shutter t] [--orbit] [--sequence n --step t --out frames/%04d.png] [--to-distance d]
  *                 [--to-tilt t] [--to-around a] [--precision fp32|fp32-fast|mixed|fp64] [--describe file]
+ *                 [--disc-on-device]
  *
  * Only the black-hole scene is here: the reference's triangle scenes are not on this backend's path.
- * --precision (beyond the reference) picks the tracer; --describe writes, for each picture written, the
- * hole's description and the camera that made it. */
+ * Beyond the reference: --precision picks the tracer; --describe writes, for each picture written, the
+ * hole's description and the camera that made it; --disc-on-device keeps the disc's map and fluid on the
+ * GPU (artfSceneSetDiscOnDevice: the same pictures, the fluid's step a frame taken off the CPU). */
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include "artful/device_disc.h"
 #include "artful/renderer.h"
 #include "artful/scene.h"
 #include "frames.h"
@@ -77,7 +80,8 @@ static void usage(void) {
     fputs("artful-render [--backend hip|auto] [--size w h] [--scene blackhole] [--frames n] [--out file.png|.ppm]\n"
           "              [--time t] [--samples n] [--doppler d] [--distance r] [--spin a] [--tilt deg] [--roll deg] [--fov deg]\n"
           "              [--exposure x] [--bloom x] [--around deg] [--shutter t] [--orbit] [--sequence n --step t]\n"
-          "              [--to-distance d] [--to-tilt t] [--to-around a] [--precision fp32|fp32-fast|mixed|fp64] [--describe file]\n",
+          "              [--to-distance d] [--to-tilt t] [--to-around a] [--precision fp32|fp32-fast|mixed|fp64] [--describe file]\n"
+          "              [--disc-on-device]\n",
           stderr);
 }
 
@@ -88,6 +92,7 @@ int main(int argc, char** argv) {
     o.noVsync = true;
     const char *out = NULL, *describe = NULL;
     int frames = 0;
+    bool onDevice = false;
     ArtfHoleTracing tracing = artfHoleTracingDefaults();
     HoleOptions hole = {0, 0, 74.1f, 3.44f, 0, 19.f, 1.f, .6f, 0, 0, -1.f, 1, false, 0, .9f, -1.f, -1000.f, -1000.f};
     for (int i = 1; i < argc; ++i) {
@@ -123,6 +128,7 @@ int main(int argc, char** argv) {
         else if (!strcmp(a, "--around") && i + 1 < argc) hole.around = (float)atof(argv[++i]);
         else if (!strcmp(a, "--shutter") && i + 1 < argc) hole.shutter = (float)atof(argv[++i]);
         else if (!strcmp(a, "--orbit")) hole.orbit = true;
+        else if (!strcmp(a, "--disc-on-device")) onDevice = true;
         else if (!strcmp(a, "--bloom") && i + 1 < argc) hole.bloom = (float)atof(argv[++i]);
         else if (!strcmp(a, "--sequence") && i + 1 < argc) hole.sequence = atoi(argv[++i]);
         else if (!strcmp(a, "--step") && i + 1 < argc) hole.step = (float)atof(argv[++i]);
@@ -151,6 +157,7 @@ int main(int argc, char** argv) {
     FILE* described = describe ? fopen(describe, "w") : NULL;
     uint32_t* pixels = malloc((size_t)o.width * o.height * 4);
     ArtfScene* s = artfSceneNew();
+    if (onDevice) artfSceneSetDiscOnDevice(s, 0);
     const double t0 = artfNowMs();
     Made made = sceneBlackHole(s, &hole);
     printf("scene built in %.1f ms\n", artfNowMs() - t0);

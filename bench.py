@@ -373,6 +373,63 @@ def animation(native, frames=100, warmup=50):
     return out
 
 
+# Artful's published black-hole figures (its README.md:381-390, `artful-render --scene blackhole` at 1920x1080,
+# GPU frames timed to the last pixel of a run): ms a frame on an AMD Vega 8 iGPU (Vulkan) and on the Ryzen 3
+# 3200G's 4 cores (its software renderer)
+PUBLISHED_BLACKHOLE_MS = {"vega8_vulkan": 23.0, "ryzen_3200g_4_cores_software": 141.0}
+
+
+def render_scene(native, frames=300, warmup=50, width=1920, height=1080):
+    """The scene of Artful's published black-hole figures: tools/render.c's black-hole scene with its
+    defaults (render.c:398: the eye 74.1 masses away and 3.44 degrees above the disc, a 19 degree field of
+    view, spin .6, one ray a pixel, no Doppler; render.c:335-360 places the eye), at 1920x1080, its time
+    going on by .2 a frame so that each frame also takes one step of the disc's fluid (render.c:551-553),
+    and the frames timed as a run to the last pixel with the read-back taken out (render.c:545-575): with
+    the fluid stepped on the CPU (the reference's way, the same host code for every backend) and on the
+    GPU (artfSceneSetDiscOnDevice), beside the published milliseconds a frame."""
+    f32 = np.float32
+    tilt = f32(3.44) * f32(3.14159265) / f32(180.0)
+    dist = f32(74.1)
+    eye = (0.0, float(np.sin(tilt, dtype=f32) * dist), float(np.cos(tilt, dtype=f32) * dist))
+    fov = float(f32(19.0) * f32(3.14159265) / f32(180.0))
+    cam = native.camera(eye, (0, 0, 0), (0, 1, 0), fov, .1)
+    r = native.Renderer(width, height)
+    r.set_tracing("fp32")
+    out = {"width": width, "height": height, "rays": width * height, "frames": frames, "warmup": warmup,
+           "published_ms": PUBLISHED_BLACKHOLE_MS}
+    last = {}
+    for where in ("host", "device"):
+        s = native.Scene(cam=cam)
+        if where == "device":
+            s.set_disc_on_device(0)
+        desc = native.defaults()
+        desc.time, desc.doppler, desc.samples, desc.exposure, desc.spin, desc.shutter = 0.0, 0.0, 1, 1.0, .6, 0.0
+        s.set_hole(desc)
+        t = f32(0.0)
+        for k in range(warmup + frames):
+            if k == warmup:
+                r.read_pixels()  # the warm-up frames done: the run starts from an idle queue
+                t0 = time.perf_counter()
+            t = t + f32(.2)
+            desc.time = float(t)
+            s.set_hole(desc)
+            if not r.draw(s):
+                raise RuntimeError("draw failed")
+        r.read_pixels()
+        wall = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        last[where] = r.read_pixels()
+        wall -= time.perf_counter() - t1
+        st = r.hole_stats()
+        out[where] = {"ms_per_frame": wall / frames * 1e3, "gpu_ms_last_frame": st.traceMs + st.resolveMs + st.bloomMs + st.toneMs}
+        s.close()
+        log(f"render scene {where}: {out[where]}")
+    r.close()
+    out["same_frames"] = bool(np.array_equal(last["host"], last["device"]))
+    out["vs_published"] = {k: v / out["device"]["ms_per_frame"] for k, v in PUBLISHED_BLACKHOLE_MS.items()}
+    return out
+
+
 EXTENDED_LAW = dict(disc="analytic", emissivity=3.0, jitter=2, bloom_levels=5)
 
 
@@ -496,7 +553,7 @@ def main():
     ap.add_argument("--profile", action="store_true", help="only the FP32 frames and the FP32 kernel timing (for rocprofv3 runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
-    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge", "extended"], help="run one extra leg alone and print its JSON")
+    ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge", "extended", "render"], help="run one extra leg alone and print its JSON")
     ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
@@ -521,7 +578,7 @@ def main():
     if args.only:
         sched = tuple(int(v) for v in args.batch_schedule.split(",")) if args.batch_schedule else None
         legs = {"batch": lambda n: batched_views(n, schedule=sched), "study": precision_study, "adjoint": adjoint_fit,
-                "animation": animation, "huge": huge_frame, "extended": extended_form}
+                "animation": animation, "huge": huge_frame, "extended": extended_form, "render": render_scene}
         leg = legs[args.only](native)
         print(json.dumps({args.only: leg}), flush=True)
         return
@@ -571,6 +628,7 @@ def main():
     renderer.close()
     scene.close()
     anim = animation(native)
+    published = render_scene(native)
     huge = huge_frame(native)
     batch = None if args.no_batch else batched_views(native)
     study = None if args.no_study else precision_study(native)
@@ -597,6 +655,7 @@ def main():
         "adjoint_over_forward": adjoint["adjoint_over_forward"],
         "adjoint": adjoint,
         "animation": anim,
+        "render_scene": published,
         "huge_frame": huge,
         "batched_views": batch,
         "precision_study": study,
