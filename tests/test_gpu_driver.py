@@ -7,7 +7,7 @@ eye's place, its orbital speed and the time are checked against the drivers' clo
 (tools/blackhole.c:35-44, tools/render.c:336-360, with the orbital speed of a circular Kerr orbit
 against the local static observers), and each picture must equal the oracle's render of that scene,
 replayed on the host (the hole set at time 0, then moved on to the frame's time, as the drivers do), byte
-for byte. The disc on the host or on the device gives the same frames."""
+for byte. The disc on the host or on the device gives the same frames, in either driver."""
 from __future__ import annotations
 
 import math
@@ -142,6 +142,11 @@ def test_render_driver_picture_and_timing(gpu, oracle, tmp_path):
     np.testing.assert_allclose(vec(desc.velocity), [math.cos(around) * speed, 0, -math.sin(around) * speed], atol=2e-6)
     assert abs(desc.time - .6) < 1e-5 and abs(cam.fovY - math.radians(19)) < 1e-6 and desc.samples == 2
     np.testing.assert_array_equal(read_image(tmp_path / "hole.png"), oracle_rgb(gpu, oracle, desc, cam, w, h))
+    # the disc's map and fluid on the GPU: the same picture after the same timed frames
+    text = run(RENDER, "--backend", "hip", "--scene", "blackhole", "--size", w, h, "--samples", 2, "--shutter", .5, "--orbit",
+               "--around", 30, "--doppler", 1, "--frames", 3, "--out", tmp_path / "hole_device.png", "--disc-on-device")
+    assert "3 frames:" in text
+    np.testing.assert_array_equal(read_image(tmp_path / "hole_device.png"), read_image(tmp_path / "hole.png"))
 
 
 def test_render_driver_sequence(gpu, oracle, tmp_path):
