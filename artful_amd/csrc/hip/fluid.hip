@@ -8,9 +8,12 @@
  *   - seed: the starting clumps (fluid.c:286-305): wrapped value-noise octaves, wound by the shear;
  *   - stir: a list of eddies applied in their order to each cell; a cell finds the one offset di of an
  *     eddy's sweep that lands on it, so it sees the same terms the host's loop over di gives it;
- *   - velocity, density, shift (a ring turned by its orbit with a clamped Catmull-Rom cubic): per cell;
- *   - balance: each ring's sum of 1024 floats taken by one lane in cell order (the host's order; a tree
- *     would round differently), then the ring scaled by every lane;
+ *   - velocity, density: per cell;
+ *   - shift and balance, a workgroup a ring: the ring turned by its orbit with a clamped Catmull-Rom
+ *     cubic, its density kept in LDS, summed there by one lane in cell order (the host's order; a tree
+ *     would round differently), then scaled by every lane. (As two passes, the shift writing the density
+ *     and the balance's one lane reading it back from memory cell by cell, the balance alone took 23 us
+ *     of a step's ~50.)
  *   - field: what shows.
  * Not in the reference (which steps its fluid on the CPU for every backend). */
 #include <hip/hip_runtime.h>
@@ -71,28 +74,28 @@ __global__ __launch_bounds__(256) void densityKernel(ArtfHipFluid f) {
     f.scratch[0][(size_t)j * FW + i] = fluidDensityCell(&q, f.density, f.scratch[1], f.scratch[2], i, j);
 }
 
-/* Each ring turned along itself by its orbit's cells a step, from the step's second buffers back into
- * the state. */
-__global__ __launch_bounds__(256) void shiftKernel(ArtfHipFluid f) {
-    int i, j;
-    if (!cellOf(i, j)) return;
+/* A workgroup a ring (fluid.c:153-161, 221-229): the ring turned along itself by its orbit's cells a
+ * step, from the step's second buffers back into the state, its density through LDS; one lane sums the
+ * turned density in cell order, then every lane scales four cells by the ring's balance. */
+__global__ __launch_bounds__(256) void ringKernel(ArtfHipFluid f) {
+    __shared__ __align__(16) float turned[FW];
+    __shared__ float scale;
+    const int j = (int)blockIdx.x;
     const float cells = f.rings->shiftCells[j];
     const size_t row = (size_t)j * FW;
-    float* const outs[3] = {f.density, f.radial, f.around};
-    for (int q = 0; q < 3; ++q) outs[q][row + i] = fluidShiftCell(f.scratch[q] + row, i, cells);
-}
-
-/* A block a ring: one lane sums it in cell order, then every lane scales four cells. */
-__global__ __launch_bounds__(256) void balanceKernel(ArtfHipFluid f) {
-    __shared__ float scale;
-    float* s = f.density + (size_t)blockIdx.x * FW;
+    for (int i = (int)threadIdx.x; i < FW; i += (int)blockDim.x) {
+        turned[i] = fluidShiftCell(f.scratch[0] + row, i, cells);
+        f.radial[row + i] = fluidShiftCell(f.scratch[1] + row, i, cells);
+        f.around[row + i] = fluidShiftCell(f.scratch[2] + row, i, cells);
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         float sum = 0;
-        for (int i = 0; i < FW; ++i) sum += s[i];
+        for (int i = 0; i < FW; ++i) sum += turned[i];
         scale = fluidBalanceScale(sum);
     }
     __syncthreads();
-    for (int i = (int)threadIdx.x; i < FW; i += (int)blockDim.x) s[i] = sharedClamp(s[i] * scale, .03f, 8.f);
+    for (int i = (int)threadIdx.x; i < FW; i += (int)blockDim.x) f.density[row + i] = sharedClamp(turned[i] * scale, .03f, 8.f);
 }
 
 __global__ __launch_bounds__(256) void fieldKernel(ArtfHipFluid f) {
@@ -120,8 +123,7 @@ hipError_t launchFluidStep(const ArtfHipFluid& f, const ArtfFluidEddy* eddies, h
     hipLaunchKernelGGL(stirKernel, dim3(kCellBlocks), dim3(256), 0, stream, f, eddies, ARTF_FLUID_STEP_EDDIES);
     hipLaunchKernelGGL(velocityKernel, dim3(kCellBlocks), dim3(256), 0, stream, f);
     hipLaunchKernelGGL(densityKernel, dim3(kCellBlocks), dim3(256), 0, stream, f);
-    hipLaunchKernelGGL(shiftKernel, dim3(kCellBlocks), dim3(256), 0, stream, f);
-    hipLaunchKernelGGL(balanceKernel, dim3((unsigned)FH), dim3(256), 0, stream, f);
+    hipLaunchKernelGGL(ringKernel, dim3((unsigned)FH), dim3(256), 0, stream, f);
     return hipGetLastError();
 }
 
