@@ -90,8 +90,14 @@ __global__ __launch_bounds__(256) void ringKernel(ArtfHipFluid f) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        /* the chain of additions in cell order, the LDS reads four cells at a time and issued ahead of it */
+        const float4* quads = reinterpret_cast<const float4*>(turned);
         float sum = 0;
-        for (int i = 0; i < FW; ++i) sum += turned[i];
+#pragma unroll 16
+        for (int i = 0; i < FW / 4; ++i) {
+            const float4 v = quads[i];
+            sum += v.x, sum += v.y, sum += v.z, sum += v.w;
+        }
         scale = fluidBalanceScale(sum);
     }
     __syncthreads();
