@@ -7,8 +7,8 @@
  * the host paints with, from its coordinates alone (a lattice cell below zero goes through a signed
  * conversion there, as on x86). One place needs care of its own to give the host's bits:
  *   - the normalising sum over 1,048,576 texels is taken in doubles in texel order, as the host's
- *     loop does: the whole block stages 4096 texels at a time in LDS and one lane adds them in order
- *     (about 3 ms; a tree would round differently).
+ *     loop does: one lane adds them in order from LDS while the block's other waves stage the next
+ *     4096 (a tree would round differently).
  * Not in the reference (which paints on the CPU, 0.6 s). */
 #include <hip/hip_runtime.h>
 
@@ -29,26 +29,35 @@ __global__ __launch_bounds__(256) void gasKernel(float* map) {
 }
 
 /* The sum of the inner half's texels in doubles, in texel order (disc.c artfDiscPaint), and the gas
- * scaled to make their mean 1. One block: its lanes stage a run of texels in LDS, lane 0 adds them in
- * order; then every lane scales (one block moving 8 MB: ~0.1 ms beside the sum's ~3 ms). */
+ * scaled to make their mean 1. One block: lane 0 adds each run of texels from LDS in order, four at a
+ * read issued ahead of the chain, while the other three waves stage the next run in the other half of
+ * LDS; then every lane scales. */
 constexpr int kRun = 4096;
 
 __global__ __launch_bounds__(256) void gasScaleKernel(float* map) {
-    __shared__ float run[kRun];
+    __shared__ __align__(16) float run[2][kRun];
     __shared__ float scale;
-    const int inner = MW * (MH / 2);
+    const int inner = MW * (MH / 2), t = (int)threadIdx.x;
+    for (int i = t; i < kRun; i += (int)blockDim.x) run[0][i] = map[i];
+    __syncthreads();
     double sum = 0;
-    for (int base = 0; base < inner; base += kRun) {
-        for (int i = (int)threadIdx.x; i < kRun; i += (int)blockDim.x) run[i] = map[base + i];
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int i = 0; i < kRun; ++i) sum += run[i];
+    for (int base = 0, b = 0; base < inner; base += kRun, b ^= 1) {
+        if (t == 0) {
+            const float4* quads = reinterpret_cast<const float4*>(run[b]);
+#pragma unroll 8
+            for (int i = 0; i < kRun / 4; ++i) {
+                const float4 v = quads[i];
+                sum += v.x, sum += v.y, sum += v.z, sum += v.w;
+            }
+        } else if (t >= 64 && base + kRun < inner) {
+            for (int i = t - 64; i < kRun; i += (int)blockDim.x - 64) run[b ^ 1][i] = map[base + kRun + i];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) scale = (float)((double)inner / sum);
+    if (t == 0) scale = (float)((double)inner / sum);
     __syncthreads();
     const float k = scale;
-    for (int i = (int)threadIdx.x; i < MW * MH; i += (int)blockDim.x) map[i] *= k;
+    for (int i = t; i < MW * MH; i += (int)blockDim.x) map[i] *= k;
 }
 
 __global__ __launch_bounds__(256) void dustKernel(float* dust) {
