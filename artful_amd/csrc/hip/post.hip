@@ -139,10 +139,12 @@ __global__ __launch_bounds__(kPost) ARTF_POST_REGISTERS void bloomUpKernel(Plane
  * into LDS, levels first+1..last are built down and levels last-1..first+1 brought up there, with the
  * arithmetic of bloomDownKernel and bloomUpKernel, and stored: the pyramid the launches level by level
  * leave, in one launch instead of 2 (last - first) - 1. Levels first..last are packed one after
- * another; together they hold at most kSmallFloats / 3 texels, and first+1..last at most kSmallWork
- * (one wave does their work: as every kernel here, it finds room as soon as one trace wave ends, so it
- * runs beside the next frame's trace). */
-constexpr int kSmallThreads = kPost;
+ * another; together they hold at most kSmallFloats / 3 texels, and first+1..last at most kSmallWork.
+ * It runs only where nothing runs beside it (a frame on one stream, artfHipBloom): beside a trace the
+ * pyramid goes level by level in one-wave workgroups, since this one's 128 KB of LDS and four waves would
+ * wait for the trace to drain. Four waves share each level's work (with one wave it took 27 us a frame
+ * on average over the GPU tests' frames, 45 us at 1080p). */
+constexpr int kSmallThreads = 256;
 constexpr int kSmallFloats = 32768; /* 128 KB of the compute unit's 160 KB */
 constexpr int kSmallWork = 1024;
 
