@@ -461,6 +461,22 @@ def extended_form(native, renderer=None, scene=None, steps=100, warmup=30):
         wall = time.perf_counter() - t0
         out[p] = {"ms_per_frame": wall / steps * 1e3, "rays_per_s": st.rays * steps / wall, "bloom_levels": st.bloomLevels,
                   "error_vs_fp64": rel_error(renderer.read_hdr(), ref64)}
+    # the same law with the sky's stars off: the disc and the dust of the sky are smooth in the ray, so each
+    # precision's error is its stepping's, with no hashed star cell for a ray's last bits to land in or miss
+    smooth_desc, cam = workload_scene(native)
+    smooth_desc.stars = 0.0
+    smooth = native.Scene(smooth_desc, cam)
+    renderer.set_tracing("fp64", **EXTENDED_LAW)
+    if not renderer.draw(smooth):
+        raise RuntimeError("draw failed")
+    ref64 = renderer.read_hdr().astype(np.float64)
+    out["without_stars"] = {}
+    for p in ("fp32", "mixed"):
+        renderer.set_tracing(p, **EXTENDED_LAW)
+        if not renderer.draw(smooth):
+            raise RuntimeError("draw failed")
+        out["without_stars"][p] = rel_error(renderer.read_hdr(), ref64)
+    smooth.close()
     renderer.set_tracing("fp32")
     if own:
         renderer.close()
