@@ -521,7 +521,8 @@ def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, dr
 def precision_study(native, width=3840, height=2160):
     """The precision study: 3840x2160, one ray a pixel, spins {0, .5, .9, .998} x inclinations {17, 45,
     75, 89}, eye at r 50, painted disc ISCO..20, Doppler 1: FP32, mixed and FP64 traces, each image's
-    relative HDR error against its FP64 trace, and rays/s."""
+    relative HDR error against its FP64 trace, and rays/s; then the same views made smooth in the ray
+    (the analytic disc, no stars), each precision's error against FP64 again."""
     from artful_amd.workloads import view_of
     r = native.Renderer(width, height)
     scene = native.Scene()
@@ -551,11 +552,30 @@ def precision_study(native, width=3840, height=2160):
             row["mixed_over_fp64"] = row["mixed_rays_per_s"] / row["fp64_rays_per_s"]
             rows.append(row)
             log(f"precision study: {row}")
+    # the same 16 views made smooth in the ray (the analytic r^-3 disc, the stars off): each precision's
+    # error is then its stepping's alone, with no texel row or hashed star cell for a ray's last bits to pick
+    smooth = []
+    for spin in (0.0, .5, .9, .998):
+        for incl in (17, 45, 75, 89):
+            desc, cam, _, _ = view_of(spin, incl, 50.0, 20.0, width, height, fov_y=.35)
+            desc.stars = 0.0
+            scene.set_hole(desc)
+            scene.set_camera(cam)
+            hdr = {}
+            for p in ("fp64", "fp32", "mixed"):
+                r.set_tracing(p, disc="analytic", emissivity=3.0)
+                if not r.draw(scene):
+                    raise RuntimeError("draw failed")
+                hdr[p] = r.read_hdr()
+            ref = hdr["fp64"].astype(np.float64)
+            smooth.append({"spin": spin, "inclination": incl, **{f"{p}_error": rel_error(hdr[p], ref) for p in ("fp32", "mixed")}})
     r.close()
     scene.close()
     summary = {p: {"max_of_max": max(x[f"{p}_error"]["max"] for x in rows), "max_of_mean": max(x[f"{p}_error"]["mean"] for x in rows),
-                   "max_share_over_1e-3": max(x[f"{p}_error"]["share_over_1e-3"] for x in rows)} for p in ("fp32", "mixed")}
-    return {"width": width, "height": height, "images": rows, "summary": summary}
+                   "max_share_over_1e-3": max(x[f"{p}_error"]["share_over_1e-3"] for x in rows),
+                   "smooth_max_of_mean": max(x[f"{p}_error"]["mean"] for x in smooth),
+                   "smooth_max_share_over_1e-3": max(x[f"{p}_error"]["share_over_1e-3"] for x in smooth)} for p in ("fp32", "mixed")}
+    return {"width": width, "height": height, "images": rows, "smooth_images": smooth, "summary": summary}
 
 
 def main():
