@@ -136,6 +136,7 @@ class HoleBatchStats(C.Structure):
 
 # enum values (include/artful/renderer.h, include/artful/hole.h)
 BACKEND_AUTO, BACKEND_VULKAN, BACKEND_SOFTWARE, BACKEND_METAL, BACKEND_HIP = range(5)
+HOLE_WORKGROUP_A_TILE = -1  # ARTF_HOLE_WORKGROUP_A_TILE: the batch with a workgroup a tile, no queue
 PRECISION = {"fp32": 0, "fp32_fast": 1, "mixed": 2, "fp64": 3}
 DISC = {"painted": 0, "analytic": 1}
 

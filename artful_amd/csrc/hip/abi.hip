@@ -279,9 +279,8 @@ static bool batchFits(const ArtfHipBatchArgs* a) {
         return false;
     if (a->viewCount > 65536) return false;
     const ArtfHipSchedule& s = a->schedule;
-    if (s.workgroups < 1 || s.workgroups > 65536 || s.chunkTiles < 1 || s.chunkTiles > 64 || s.refill < 1 || s.refill > 64 || s.stepsPerCheck < 1 ||
-        s.stepsPerCheck > 16)
-        return false;
+    if (s.workgroups != ARTF_HIP_WORKGROUP_A_TILE && (s.workgroups < 1 || s.workgroups > 65536)) return false;
+    if (s.chunkTiles < 1 || s.chunkTiles > 64 || s.refill < 1 || s.refill > 64 || s.stepsPerCheck < 1 || s.stepsPerCheck > 16) return false;
     return a->precision >= ARTF_PRECISION_FP32 && a->precision <= ARTF_PRECISION_FP64;
 }
 

@@ -60,8 +60,8 @@ __device__ __forceinline__ void mixedStep(const RayConst<double>& cd, const RayC
     }
 }
 
-__device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const ArtfHoleFrame64& f64, const TraceLaw& law, const float* __restrict__ map,
-                                   const float* __restrict__ fluid, double nx, double ny, float time) {
+__device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const ArtfHoleFrame64& f64, const TraceLaw& law, const DiscTables& tables,
+                                                   double nx, double ny, float time) {
     Eye<double> e;
     eyeFrom64(f64, e);
     RayConst<double> cd;
@@ -83,8 +83,8 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
         StepLooks<float> looks;
         mixedStep(cd, cf, s, uHorizon, n, looks);
         if (s.n2 * n.n2 < 0.f) {
-            if (law.disc == DISC_PAINTED) crossPainted(d, cf, s, n, looks, tablesOf(map, fluid), acc);
-            else crossAnalytic(d, cf, s, n, looks, law.q, map, acc);
+            if (law.disc == DISC_PAINTED) crossPainted(d, cf, s, n, looks, tables, acc);
+            else crossAnalytic(d, cf, s, n, looks, law.q, tables.map, acc);
         }
         const bool escaped = uEscape > n.u;
         if (escaped || n.u > uHorizon || !(acc.through > .004f)) { /* where it left, only as it leaves: as traceRay */
@@ -101,7 +101,7 @@ __device__ __forceinline__ RayResult traceRayMixed(const ArtfHoleFrame& f, const
         }
         s = n;
     }
-    if (f.look[2] > 0.f && sky) skyLight(f.look[2], cf, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
+    if (f.look[2] > 0.f && sky) skyLight(f.look[2], cf, awayX, awayY, awayZ, awayPsi, d.doppler, tables.map, acc);
     RayResult out;
     out.r = acc.r, out.g = acc.g, out.b = acc.b, out.steps = steps;
     return out;

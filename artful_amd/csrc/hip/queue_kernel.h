@@ -1,6 +1,7 @@
-/* artful-hip — the persistent, work-queued tracer for a batch of views.
+/* artful-hip — the tracers for a batch of views: a workgroup a tile (viewTilesKernel, at the end) or
+ * persistent and work-queued (queueKernel).
  *
- * A fixed grid of workgroups stays on the device until every ray of every view has ended. Each wave
+ * The queued kernel: a fixed grid of workgroups stays on the device until every ray of every view has ended. Each wave
  * (64 lanes, one ray a lane) takes chunks of 8 x 8-ray tiles from a queue (a counter, one atomic add
  * per chunk), keeps them in a wave-private pool, and refills: once at least
  * schedule.refill of its lanes are idle, it finishes their rays (the sky's light and the write, all
@@ -362,8 +363,74 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
     }
 }
 
-/* The batch's kernel, with the drag's quotient in range where every view's spin allows it. */
+/* The batch with no queue (schedule.workgroups = ARTF_HIP_WORKGROUP_A_TILE): a workgroup of one wave for
+ * each 8x8 tile of rays, as the frame's tile kernel has, and the hardware hands the workgroups to compute
+ * units as their slots free up. A tile is one view's, so the view's frame, ray law and tables are wave-uniform loads (scalar, from
+ * the restrict arguments), and each ray is traced by the tile kernel's own code (tracePixelRay) at its
+ * registers and waves a SIMD: no pool, queue, LDS copy of the view or refill. It traces what the queued
+ * kernel does with refill 64 (new rays only once a wave's 64 lanes are idle), the same tiles the same
+ * way, bit for bit; the tuner weighs the two. Workgroup b goes to XCD b % 8, and each XCD takes four
+ * neighbouring tiles of every 32, as in traceKernel. */
+template <int Precision, bool InRangeDrag>
+__global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(Precision)))) void viewTilesKernel(
+    const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile, ArtfHipBatchArgs a) {
+    const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
+    const uint32_t tile = a.tileBegin + ((slot >> 2) * 8u + xcd) * 4u + (slot & 3u);
+    if (tile >= a.tileEnd) return; /* the grid rounded up to 32: the whole wave leaves */
+    const uint32_t view = viewOfTile(firstTile, a.viewCount, tile);
+    const ArtfHipView& v = views[view];
+    const ArtfHoleFrame& f = v.frame;
+    const int lane = (int)threadIdx.x;
+    /* the view's tiles sub-ray by sub-ray, rows of tiles top to bottom; the lane's ray in the 8x8 tile */
+    const uint32_t local = tile - firstTile[view], perSub = v.tilesX * v.tilesY;
+    const uint32_t sub = local / perSub, rest = local - sub * perSub;
+    const uint32_t ty = rest / v.tilesX, tx = rest - ty * v.tilesX;
+    const int px = (int)(tx * 8u) + (lane & 7), py = (int)(ty * 8u) + (lane >> 3);
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
+    const bool valid = px < width && py < height;
+
+    TraceLaw law;
+    law.maxSteps = v.maxSteps;
+    law.uEscape = v.escapeRadius > 0.f ? 1.f / v.escapeRadius : 0.f;
+    law.disc = v.disc;
+    law.q = v.emissivity;
+    law.inRangeDrag = InRangeDrag;
+    DiscTables tables;
+    tables.map = a.map;
+    tables.rings = a.rings + (size_t)view * ARTF_HIP_RING_FLOATS;
+    tables.glow = tables.rings + ARTF_HOLE_MAP_HEIGHT * 2;
+    tables.fluid = a.fluids ? a.fluids + (size_t)view * ((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT) : a.map;
+
+    RayResult ray = {0.f, 0.f, 0.f, 0};
+    if (valid) ray = tracePixelRay<Precision>(f, v.frame64, law, tables, v.jitter, px, py, (int)(sub % (uint32_t)side), (int)(sub / (uint32_t)side), (int)sub, side);
+    /* the wave's steps, and its lane-slots as the queued kernel counts them (64 lanes a step the wave takes) */
+    if (a.counters) {
+        const uint32_t sum = waveSum((uint32_t)ray.steps), most = waveMax((uint32_t)ray.steps);
+        if (lane == 0) {
+            const unsigned at = blockIdx.x % ARTF_HIP_COUNTER_SLOTS;
+            atomicAdd(&a.counters[at], (unsigned long long)sum);
+            atomicAdd(&a.counters[ARTF_HIP_COUNTER_SLOTS + at], (unsigned long long)most * 64ull);
+        }
+    }
+    if (a.stamps && lane == 0) atomicMax(&a.stamps[1 + view * ARTF_HIP_STAMP_SLOTS + blockIdx.x % ARTF_HIP_STAMP_SLOTS], (unsigned long long)wall_clock64());
+    if (!valid) return;
+    const uint32_t outAt = (side == 1 ? 0u : sub * (uint32_t)width * (uint32_t)height) + (uint32_t)py * (uint32_t)width + (uint32_t)px;
+    v.out[0][outAt] = ray.r;
+    v.out[1][outAt] = ray.g;
+    v.out[2][outAt] = ray.b;
+}
+
+/* The batch's kernel, with the drag's quotient in range where every view's spin allows it: the queued
+ * one, or with ARTF_HIP_WORKGROUP_A_TILE the one with a workgroup a tile. */
 template <int Precision> inline hipError_t launchQueueWith(const ArtfHipBatchArgs& a, hipStream_t stream) {
+    if (a.schedule.workgroups == ARTF_HIP_WORKGROUP_A_TILE) {
+        const uint32_t tiles = a.tileEnd - a.tileBegin;
+        if (a.tileEnd <= a.tileBegin || tiles > (1u << 25)) return hipErrorInvalidValue;
+        void (*kernel)(const ArtfHipView*, const uint32_t*, ArtfHipBatchArgs) =
+            a.dragInRange ? viewTilesKernel<Precision, true> : viewTilesKernel<Precision, false>;
+        hipLaunchKernelGGL(kernel, dim3((tiles + 31u) / 32u * 32u), dim3(kTraceGroup), 0, stream, a.views, a.firstTile, a);
+        return hipGetLastError();
+    }
     void (*kernel)(const ArtfHipView*, const uint32_t*, ArtfHipBatchArgs) =
         a.dragInRange ? queueKernel<Precision, true> : queueKernel<Precision, false>;
     hipLaunchKernelGGL(kernel, dim3((unsigned)a.schedule.workgroups), dim3(kQueueGroup), 0, stream, a.views, a.firstTile, a);

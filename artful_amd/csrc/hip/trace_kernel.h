@@ -107,6 +107,33 @@ __host__ __device__ constexpr PixelTile pixelTileOf(int side) {
     return side == 1 ? PixelTile{8, 8} : side == 2 ? PixelTile{4, 4} : side == 3 ? PixelTile{7, 1} : side == 4 ? PixelTile{2, 2} : side == 5 ? PixelTile{2, 1} : PixelTile{1, 1};
 }
 
+/* Sub-ray (sx, sy) of pixel (px, py), sub = sy * side + sx of side^2, traced in the given precision: its
+ * place in the pixel, its moment, its ray (lens.inc:364-372). */
+template <int Precision>
+__device__ __forceinline__ RayResult tracePixelRay(const ArtfHoleFrame& f, const ArtfHoleFrame64& f64, const TraceLaw& law, const DiscTables& tables,
+                                                   uint32_t jitter, int px, int py, int sx, int sy, int sub, int side) {
+    float dx, dy;
+    subPixel(jitter, px, py, sx, sy, side, dx, dy);
+    const float time = rayMoment(f, sub, side);
+    if (Precision == ARTF_PRECISION_FP64) {
+        Eye<double> e;
+        eyeFrom64(f64, e);
+        double nx, ny;
+        aimOf(f, px, py, dx, dy, nx, ny);
+        return traceRay<double>(f, e, law, tables, nx, ny, time);
+    } else if (Precision == ARTF_PRECISION_MIXED) {
+        double nx, ny;
+        aimOf(f, px, py, dx, dy, nx, ny);
+        return traceRayMixed(f, f64, law, tables, nx, ny, time);
+    } else {
+        Eye<float> e;
+        eyeFrom<float>(f, e);
+        float nx, ny;
+        aimOf(f, px, py, dx, dy, nx, ny);
+        return traceRay<float>(f, e, law, tables, nx, ny, time);
+    }
+}
+
 /* InRangeDrag: the FP32 steps take the drag's quotient by quotInRange, for a frame whose spin allows it
  * (artfHipDragInRange); the same bits as with IEEE division, two instructions fewer for each of a step's two. */
 template <int Precision, bool InRangeDrag = false>
@@ -140,28 +167,7 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
     law.inRangeDrag = InRangeDrag;
 
     RayResult ray = {0.f, 0.f, 0.f, 0};
-    if (valid) {
-        float dx, dy;
-        subPixel(a.jitter, px, py, sx, sy, side, dx, dy);
-        const float time = rayMoment(f, sub, side);
-        if (Precision == ARTF_PRECISION_FP64) {
-            Eye<double> e;
-            eyeFrom64(a.frame64, e);
-            double nx, ny;
-            aimOf(f, px, py, dx, dy, nx, ny);
-            ray = traceRay<double>(f, e, law, a.map, a.fluid, nx, ny, time);
-        } else if (Precision == ARTF_PRECISION_MIXED) {
-            double nx, ny;
-            aimOf(f, px, py, dx, dy, nx, ny);
-            ray = traceRayMixed(f, a.frame64, law, a.map, a.fluid, nx, ny, time);
-        } else {
-            Eye<float> e;
-            eyeFrom<float>(f, e);
-            float nx, ny;
-            aimOf(f, px, py, dx, dy, nx, ny);
-            ray = traceRay<float>(f, e, law, a.map, a.fluid, nx, ny, time);
-        }
-    }
+    if (valid) ray = tracePixelRay<Precision>(f, a.frame64, law, tablesOf(a.map, a.fluid), a.jitter, px, py, sx, sy, sub, side);
     countSteps(a.counters, ray.steps, valid);
     if (count == 1) {
         if (!valid) return;

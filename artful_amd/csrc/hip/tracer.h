@@ -584,8 +584,8 @@ template <typename R> __device__ __forceinline__ void discFrom(const ArtfHoleFra
 
 /* One ray, eye to end, all in R. nx, ny: the pixel's place in -1..1; time: this ray's moment. */
 template <typename R>
-__device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<R>& e, const TraceLaw& law, const float* __restrict__ map,
-                                              const float* __restrict__ fluid, R nx, R ny, float time) {
+__device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<R>& e, const TraceLaw& law, const DiscTables& tables, R nx, R ny,
+                                              float time) {
     RayConst<R> c;
     RayState<R> s;
     setupRay(e, nx, ny, c, s);
@@ -603,8 +603,8 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
         StepLooks<R> looks;
         nystromStep(c, s, n, looks);
         if (s.n2 * n.n2 < (R)0) { /* through the disc's plane */
-            if (law.disc == DISC_PAINTED) crossPainted(d, c, s, n, looks, tablesOf(map, fluid), acc);
-            else crossAnalytic(d, c, s, n, looks, law.q, map, acc);
+            if (law.disc == DISC_PAINTED) crossPainted(d, c, s, n, looks, tables, acc);
+            else crossAnalytic(d, c, s, n, looks, law.q, tables.map, acc);
         }
         const bool escaped = uEscape > n.u;
         /* A lane leaves the loop at escape, at the horizon or opaque. Where an escaping ray left is worked
@@ -625,10 +625,17 @@ __device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<
         }
         s = n;
     }
-    if (f.look[2] > 0.f && sky) skyLight(f.look[2], c, awayX, awayY, awayZ, awayPsi, d.doppler, map, acc);
+    if (f.look[2] > 0.f && sky) skyLight(f.look[2], c, awayX, awayY, awayZ, awayPsi, d.doppler, tables.map, acc);
     RayResult out;
     out.r = (float)acc.r, out.g = (float)acc.g, out.b = (float)acc.b, out.steps = steps;
     return out;
+}
+
+/* A frame's ray: the rings and glow inside its map (the frame's map layout). */
+template <typename R>
+__device__ __forceinline__ RayResult traceRay(const ArtfHoleFrame& f, const Eye<R>& e, const TraceLaw& law, const float* __restrict__ map,
+                                              const float* __restrict__ fluid, R nx, R ny, float time) {
+    return traceRay(f, e, law, tablesOf(map, fluid), nx, ny, time);
 }
 
 } // namespace artf

@@ -1,11 +1,12 @@
-/* artful-hip — a batch of black-hole views traced by one persistent launch (include/artful/batch.h).
+/* artful-hip — a batch of black-hole views traced by one launch (include/artful/batch.h).
  *
  * Made once, on the device (include/artful/device_disc.h): the painted map's shared tables (gas,
  * colours, sky, dust) for all views; per view its clamped description, FP32 and FP64 frames, ring
  * tables and (painted disc) fluid
- * with the view table and the queue's tile offsets. Each draw replays one HIP graph: the workspace
- * reset kernel, the persistent trace over all views, and the resolve of each view with several rays a
- * pixel. The tuner times candidate schedules on the batch itself and keeps the fastest. */
+ * with the view table and the views' tile offsets. Each draw replays one HIP graph: the workspace
+ * reset kernel, the trace over all views (a workgroup a tile, or persistent and queued), and the
+ * resolve of each view with several rays a pixel. The tuner times candidate schedules on the batch
+ * itself and keeps the fastest. */
 #include <stdio.h>
 #include <string.h>
 
@@ -78,11 +79,11 @@ void artfHoleBatchFree(ArtfHoleBatch* b) {
 }
 
 /* The schedule with its zeros filled in and its fields in range. */
-static ArtfHoleSchedule resolved(const ArtfHoleBatch* b, const ArtfHoleSchedule* s) {
+static ArtfHoleSchedule resolved(const ArtfHoleSchedule* s) {
     ArtfHoleSchedule r = s ? *s : (ArtfHoleSchedule){0};
-    const int occupancy = artfMaxi(artfHipViewsOccupancy(b->tracing.precision), 1);
-    if (r.workgroups <= 0) r.workgroups = b->computeUnits * occupancy;
-    r.workgroups = artfClampi(r.workgroups, 1, 65536);
+    /* the default: a workgroup a tile, which outran every queued schedule on the batches measured */
+    if (r.workgroups <= 0) r.workgroups = ARTF_HOLE_WORKGROUP_A_TILE;
+    else r.workgroups = artfClampi(r.workgroups, 1, 65536);
     r.chunkTiles = r.chunkTiles <= 0 ? 2 : artfClampi(r.chunkTiles, 1, 64);
     r.refill = r.refill <= 0 ? 64 : artfClampi(r.refill, 1, 64);
     r.stepsPerCheck = r.stepsPerCheck <= 0 ? 8 : artfClampi(r.stepsPerCheck, 1, 16);
@@ -215,14 +216,14 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     a->stamps = (unsigned long long*)(w + 2 * queueBytes + slots * 8);
     b->countersHost = artfMalloc(slots * 8);
     b->stampsHost = artfCalloc(stamps, 8);
-    b->schedule = resolved(b, NULL);
+    b->schedule = resolved(NULL);
     return b;
 }
 
 /* The graph of one draw: the reset, the trace over all views, each view's resolve. In slices of
  * launchTiles tiles, the slices take turns on two streams, each with its own set of queues (cleared
- * before each of its slices): a slice's last waves, as the persistent kernel drains, run beside the next
- * slice's first, instead of the device idling between launches. */
+ * before each of its slices, for the queued kernel): a slice's last waves, as its launch drains, run
+ * beside the next slice's first, instead of the device idling between launches. */
 static bool capture(ArtfHoleBatch* b) {
     ArtfHipBatchArgs* a = &b->args;
     a->schedule.workgroups = b->schedule.workgroups, a->schedule.chunkTiles = b->schedule.chunkTiles;
@@ -317,7 +318,7 @@ bool artfHoleBatchLatency(const ArtfHoleBatch* b, double* ms) {
 
 bool artfHoleBatchSetSchedule(ArtfHoleBatch* b, const ArtfHoleSchedule* schedule) {
     if (!b) return false;
-    const ArtfHoleSchedule s = resolved(b, schedule);
+    const ArtfHoleSchedule s = resolved(schedule);
     if (b->graph && !memcmp(&s, &b->schedule, sizeof s)) return true;
     b->schedule = s;
     artfHipGraphFree(b->graph);
@@ -340,10 +341,11 @@ bool artfHoleBatchTune(ArtfHoleBatch* b, int rounds, ArtfHoleSchedule* best) {
     if (!b) return false;
     if (rounds <= 0) rounds = 2;
     const int occupancy = artfMaxi(artfHipViewsOccupancy(b->tracing.precision), 1), cus = b->computeUnits;
-    int32_t groups[8];
+    int32_t groups[9];
     int groupCount = 0;
     for (int k = 1; k <= occupancy && groupCount < 7; ++k) groups[groupCount++] = cus * k;
     groups[groupCount++] = cus * occupancy * 2; /* more than fit: the rest wait for a slot */
+    groups[groupCount++] = ARTF_HOLE_WORKGROUP_A_TILE;
     const int32_t chunks[] = {1, 2, 4, 8, 16};
     const int32_t refills[] = {4, 8, 16, 24, 32, 48, 64};
     const int32_t strides[] = {1, 2, 4, 8};
@@ -355,6 +357,7 @@ bool artfHoleBatchTune(ArtfHoleBatch* b, int rounds, ArtfHoleSchedule* best) {
     if (nowMs < 0) return false;
     for (int round = 0; round < rounds; ++round) {
         for (int coord = 0; coord < 4; ++coord) {
+            if (coord > 0 && now.workgroups == ARTF_HOLE_WORKGROUP_A_TILE) break; /* the queue's fields do not apply */
             for (int i = 0; i < counts[coord]; ++i) {
                 ArtfHoleSchedule trial = now;
                 int32_t* field = coord == 0 ? &trial.workgroups : coord == 1 ? &trial.chunkTiles : coord == 2 ? &trial.refill : &trial.stepsPerCheck;

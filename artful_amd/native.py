@@ -363,7 +363,7 @@ def calls_left(n: int) -> None:
 
 
 class HoleBatch:
-    """Many views traced by one persistent launch (include/artful/batch.h). views: (desc, camera, width,
+    """Many views traced by one launch (include/artful/batch.h). views: (desc, camera, width,
     height) tuples; tracing: a HoleTracing (artful_amd.native.tracing) for all of them."""
 
     def __init__(self, views, trc=None, device: int = 0):

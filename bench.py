@@ -276,7 +276,8 @@ def per_view_tiles(native, views, trc, reps=3):
 def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=None):
     """The batch workload: 64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
     U[5,85], eye r U[20,100], disc outer U[10,40], each with its own rings and fluid; rays of up to 5000
-    steps; one persistent launch replayed from a HIP graph, schedule from the tuner. Beside it, the
+    steps; one launch replayed from a HIP graph, schedule from the tuner (a workgroup a tile, or the
+    persistent queued kernel with its workgroups, chunk, refill and check interval). Beside it, the
     analytic disc through the same batch and through the tile kernel one launch per view."""
     from artful_amd.workloads import uneven_views
     views, _ = uneven_views(count, seed)
@@ -486,7 +487,7 @@ def extended_form(native, renderer=None, scene=None, steps=100, warmup=30):
 
 def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, draws=3):
     """The huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays) as a batch of one view in slices
-    of 2^20 tiles (67 M rays, ~8 ms a launch: watchdog-sized; the batch writes its rays' 6.4 GB of planes
+    of 2^20 tiles (67 M rays, ~7 ms a launch: watchdog-sized; the batch writes its rays' 6.4 GB of planes
     and resolves them), beside the renderer's single launch of the tile kernel (each pixel's 64 rays
     summed in one wave's lanes)."""
     desc, cam = workload_scene(native, samples=side)
@@ -590,7 +591,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-study", action="store_true", help="skip the 4K precision study")
     ap.add_argument("--only", choices=["batch", "study", "adjoint", "animation", "huge", "extended", "render"], help="run one extra leg alone and print its JSON")
-    ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default)")
+    ap.add_argument("--batch-schedule", help="workgroups,chunk,refill,stride for the batch leg instead of tuning (0: default; workgroups -1: a workgroup a tile)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead: a plain device-to-device copy of this many bytes, 5 times (counter calibration)")
     args = ap.parse_args()

@@ -1,8 +1,9 @@
 /* artful-hip — many views of black holes traced in one launch (beyond the reference, which draws one
  * view a frame through artfRendererDraw, renderer.h). Each view has its own description (spin, disc,
  * samples, ...), camera and size; all share a precision and ray law. The views' rays go through one
- * persistent, work-queued kernel, replayed from a HIP graph (the workspace reset, the trace, each
- * view's resolve), whose schedule artfHoleBatchTune picks by timing candidates on the batch itself.
+ * launch, replayed from a HIP graph (the workspace reset, the trace, each view's resolve): by default a
+ * workgroup of one wave for each 8x8 tile of rays, or a persistent, work-queued kernel; artfHoleBatchTune
+ * picks the schedule by timing candidates on the batch itself.
  *
  * Each view's HDR equals what the HIP renderer draws for a scene with the same description and
  * camera at the same time, bit for bit in FP32. A view's disc tables (rings, and the fluid for the
@@ -23,9 +24,15 @@ typedef struct ArtfHoleView {
     int32_t width, height;
 } ArtfHoleView;
 
-/* How the persistent launch spends the device (the first four fields as artful_hip.h's ArtfHipSchedule). */
+/* workgroups for no persistent launch: a workgroup of one wave for each 8x8 tile of rays, handed out by
+ * the hardware as the frame's tracer is (chunkTiles, refill and stepsPerCheck then do not matter) */
+#define ARTF_HOLE_WORKGROUP_A_TILE (-1)
+
+/* How the launch spends the device (the first four fields as artful_hip.h's ArtfHipSchedule). */
 typedef struct ArtfHoleSchedule {
-    int32_t workgroups;    /* 0: all the device holds at once */
+    /* 0 (the default) or below: ARTF_HOLE_WORKGROUP_A_TILE; 1..65536: that many persistent workgroups of
+     * 256 lanes take tiles from queues (artfHipViewsOccupancy of them fit a compute unit at once) */
+    int32_t workgroups;
     int32_t chunkTiles;    /* 8x8 tiles a wave takes from the queue at once */
     int32_t refill;        /* idle lanes of 64 that make a wave take new rays */
     int32_t stepsPerCheck; /* steps between two looks at the idle lanes */

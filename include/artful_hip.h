@@ -135,9 +135,12 @@ typedef struct ArtfHipView {
  * themselves; a batch says so in ArtfHipBatchArgs.dragInRange. */
 ARTF_INLINE int artfHipDragInRange(float spin) { return spin == 0.f || spin >= 0x1p-50f || spin <= -0x1p-50f; }
 
+/* schedule.workgroups for a launch of a workgroup of one wave for each tile, with no queue */
+#define ARTF_HIP_WORKGROUP_A_TILE (-1)
+
 /* How the persistent launch spends the device. */
 typedef struct ArtfHipSchedule {
-    int32_t workgroups;    /* of 256 lanes (four independent waves); at least 1 */
+    int32_t workgroups;    /* of 256 lanes (four independent waves), 1..65536; or ARTF_HIP_WORKGROUP_A_TILE */
     int32_t chunkTiles;    /* 8x8 tiles of rays a wave takes from the queue at once, 1..64 */
     int32_t refill;        /* a wave gives new rays to its idle lanes once at least this many of its 64 are idle, 1..64 */
     int32_t stepsPerCheck; /* steps between two looks at the idle lanes, 1..16 */

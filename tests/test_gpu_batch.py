@@ -104,16 +104,18 @@ def test_fp32_analytic_long_rays_equals_renderer(gpu, renderer):
     print("steps per ray", stats.steps / stats.rays, "active-lane fraction", stats.steps / stats.laneSlots)
 
 
-@pytest.mark.parametrize("schedule", [(1, 1, 64, 1), (3, 5, 1, 3), (700, 64, 17, 16), (0, 2, 8, 2), (0, 2, 64, 8, 100), (5, 3, 20, 2, 7)])
+@pytest.mark.parametrize("schedule", [(1, 1, 64, 1), (3, 5, 1, 3), (700, 64, 17, 16), (1024, 2, 8, 2), (1024, 2, 64, 8, 100), (5, 3, 20, 2, 7),
+                                      (0, 0, 0, 0), (abi.HOLE_WORKGROUP_A_TILE, 1, 1, 1), (abi.HOLE_WORKGROUP_A_TILE, 2, 64, 8, 7)])
 def test_schedules_give_the_same_image(gpu, renderer, schedule):
-    """One workgroup alone, odd chunks and refill thresholds, more workgroups than rays, the batch in
-    slices of 100 and of 7 tiles (one launch each, queues cleared between)."""
+    """The queued kernel: one workgroup alone, odd chunks and refill thresholds, more workgroups than rays,
+    the batch in slices of 100 and of 7 tiles (one launch each, queues cleared between); the kernel with a
+    workgroup a tile, the default, whole and in slices of 7."""
     tracing = dict(precision="fp32", disc="painted")
     images, stats = batch_images(gpu, SMALL, schedule=schedule, **tracing)
     for k, view in enumerate(SMALL):
         assert same_bits(images[k], rendered(gpu, renderer, view, **tracing)), f"view {k} under {schedule}"
-    if schedule[0]:
-        assert stats.schedule.workgroups == schedule[0]
+    assert stats.schedule.workgroups == (schedule[0] or abi.HOLE_WORKGROUP_A_TILE)
+    assert stats.steps > 0 and stats.steps <= stats.laneSlots
 
 
 def test_graph_replays(gpu, renderer):
@@ -136,13 +138,14 @@ def test_graph_replays(gpu, renderer):
     b.close()
 
 
-def test_graph_replays_in_slices(gpu, renderer):
-    """The batch in slices of 7 tiles, which take turns on two streams with a set of queues each: every
-    replay of its graph, with other device work between them, gives the renderer's images and the same
-    counts."""
+@pytest.mark.parametrize("workgroups", [abi.HOLE_WORKGROUP_A_TILE, 1024])
+def test_graph_replays_in_slices(gpu, renderer, workgroups):
+    """The batch in slices of 7 tiles, which take turns on two streams (with a set of queues each, for the
+    queued kernel): every replay of its graph, with other device work between them, gives the renderer's
+    images and the same counts."""
     tracing = dict(precision="fp32", disc="painted")
     b = gpu.HoleBatch(SMALL, gpu.tracing(**tracing))
-    b.set_schedule(0, 2, 64, 8, 7)
+    b.set_schedule(workgroups, 2, 64, 8, 7)
     counts = None
     for replay in range(3):
         b.draw()
@@ -160,7 +163,7 @@ def test_tune_keeps_the_image(gpu, renderer):
     tracing = dict(precision="fp32", disc="painted")
     b = gpu.HoleBatch(SMALL, gpu.tracing(**tracing))
     best = b.tune(rounds=1)
-    assert best.workgroups >= 1 and 1 <= best.chunkTiles <= 64 and 1 <= best.refill <= 64 and 1 <= best.stepsPerCheck <= 16
+    assert (best.workgroups >= 1 or best.workgroups == abi.HOLE_WORKGROUP_A_TILE) and 1 <= best.chunkTiles <= 64 and 1 <= best.refill <= 64 and 1 <= best.stepsPerCheck <= 16
     b.draw()
     s = b.stats()
     assert (s.schedule.workgroups, s.schedule.chunkTiles, s.schedule.refill, s.schedule.stepsPerCheck) == \

@@ -88,10 +88,16 @@ def test_precision_study_4k(spin, gpu, renderer4k, oracle):
 def test_batch_of_64_uneven_views_against_oracle(gpu, oracle):
     views, params = uneven_views(64, 4)
     b = gpu.HoleBatch(views, gpu.tracing(precision="fp32", disc="painted", max_steps=5000))
-    b.draw()
+    b.draw()  # the default schedule: a workgroup a tile
     st = b.stats()
     images = [b.read_hdr(v) for v in range(len(views))]
+    b.set_schedule(1024, 1, 16, 4)  # the queued kernel, idle lanes refilled 16 at a time
+    b.draw()
+    queued = b.stats()
+    for k in range(len(views)):
+        np.testing.assert_array_equal(b.read_hdr(k).view(np.uint32), images[k].view(np.uint32), err_msg=f"view {k}, queued")
     b.close()
+    assert queued.steps == st.steps
     rays = sum(w * h * d.samples ** 2 for d, _, w, h in views)
     assert st.views == 64 and st.rays == rays and 0 < st.steps <= st.laneSlots
     assert 10 < st.steps / st.rays < 60, st.steps / st.rays

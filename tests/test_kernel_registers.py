@@ -1,6 +1,6 @@
 """The register budgets the frame path relies on, read from the kernels' own compiled metadata (hipcc
-for gfx950, device code only, the library's flags): the exact FP32 tracer in at most 96 VGPRs, so that a
-SIMD runs 5 of its waves, with at most two registers spilled (a value held across the step loop, stored
+for gfx950, device code only, the library's flags): the exact FP32 tracer, and the batch's kernel with a
+workgroup a tile, which traces with its code, in at most 96 VGPRs, so that a SIMD runs 5 of their waves, with at most two registers spilled (a value held across the step loop, stored
 once as the ray starts and read once as it ends: 8 bytes of scratch a lane, none of it in the loop); and the post kernels a frame runs beside the next frame's
 trace (bloom level 0, a level down, a level up, the tone map) in at most 32, the registers a SIMD then has
 left, so that they start as soon as a trace wave ends. A compiler that gave any of them more would leave
@@ -42,6 +42,10 @@ def one(meta, part):
 def test_exact_tracer_fits_five_waves(tmp_path):
     meta = kernels("hip/trace_exact.hip", "off", B.ILP, tmp_path)
     for part in ("traceKernelILi0ELb0E", "traceKernelILi0ELb1E"):  # the drag's quotient by IEEE division, and in range
+        vgprs, spills = one(meta, part)
+        assert vgprs <= 96 and spills <= 2, (part, vgprs, spills)
+    # the batch's kernel with a workgroup a tile traces with the same code at the same budget
+    for part in ("viewTilesKernelILi0ELb0E", "viewTilesKernelILi0ELb1E"):
         vgprs, spills = one(meta, part)
         assert vgprs <= 96 and spills <= 2, (part, vgprs, spills)
 
