@@ -103,9 +103,9 @@ class HipAdjointArgs(C.Structure):
 
 
 class HipView(C.Structure):
-    _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("out", C.c_void_p * 3), ("maxSteps", C.c_int32), ("escapeRadius", C.c_float),
-                ("disc", C.c_int32), ("emissivity", C.c_float), ("jitter", C.c_uint32), ("tilesX", C.c_uint32),
-                ("tilesY", C.c_uint32), ("rays", C.c_uint32)]
+    _fields_ = [("frame", HoleFrame), ("frame64", HoleFrame64), ("hdr", C.c_void_p * 3), ("rays", C.c_void_p * 3), ("maxSteps", C.c_int32),
+                ("escapeRadius", C.c_float), ("disc", C.c_int32), ("emissivity", C.c_float), ("jitter", C.c_uint32), ("tilesX", C.c_uint32),
+                ("tilesY", C.c_uint32), ("rayCount", C.c_uint32)]
 
 
 class HipSchedule(C.Structure):
@@ -154,4 +154,4 @@ DUST_AT = GLOW_AT + MAP_HEIGHT
 MAP_FLOATS = DUST_AT + DUST_WIDTH * DUST_HEIGHT
 
 assert C.sizeof(BlackHoleDesc) == 92 and C.sizeof(Camera) == 44 and C.sizeof(HoleFrame) == 368
-assert C.sizeof(HipView) == 776 and C.sizeof(HoleView) == 144
+assert C.sizeof(HipView) == 800 and C.sizeof(HoleView) == 144

@@ -402,7 +402,7 @@ hipError_t launchQueueReset(const ArtfHipBatchArgs& a, hipStream_t stream) {
 }
 
 hipError_t launchBatchReset(const ArtfHipBatchArgs& a, hipStream_t stream) {
-    const int stamps = 1 + a.viewCount * ARTF_HIP_STAMP_SLOTS;
+    const int stamps = (int)ARTF_HIP_STAMP_WORDS(a.viewCount);
     const int n = max(2 * ARTF_HIP_COUNTER_SLOTS, stamps);
     hipLaunchKernelGGL(batchResetKernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, a.queue, a.counters, a.stamps, stamps);
     return hipGetLastError();

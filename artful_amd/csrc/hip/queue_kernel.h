@@ -1,15 +1,18 @@
 /* artful-hip — the tracers for a batch of views: a workgroup a tile (viewTilesKernel, at the end) or
  * persistent and work-queued (queueKernel).
  *
- * The queued kernel: a fixed grid of workgroups stays on the device until every ray of every view has ended. Each wave
- * (64 lanes, one ray a lane) takes chunks of 8 x 8-ray tiles from a queue (a counter, one atomic add
+ * A tile is 64 rays of one view: its pixels' rays in adjacent lanes, as the frame's tile kernel has them
+ * (ARTF_HIP_TILE_W x ARTF_HIP_TILE_H pixels of side^2 rays; 8x8 pixels at one ray a pixel).
+ *
+ * The queued kernel: a fixed grid of workgroups stays on the device until every ray of every view has
+ * ended. Each wave (64 lanes, one ray a lane) takes chunks of tiles from a queue (a counter, one atomic add
  * per chunk), keeps them in a wave-private pool, and refills: once at least
  * schedule.refill of its lanes are idle, it finishes their rays (the sky's light and the write, all
  * those lanes together), then hands the idle lanes the pool's next rays, in lane order (a ballot and
  * a prefix count over the idle lanes), and sets them up.
  *
  * A wave holds the rays of one view at a time: its view is a wave-uniform value, and on each change
- * of view the wave copies the view's entry (frames, ray law, outputs: 776 bytes) into its own slot of
+ * of view the wave copies the view's entry (frames, ray law, outputs: 800 bytes) into its own slot of
  * LDS, from where every lane reads it by broadcast. The per-lane state is then only the ray's own,
  * and a disc crossing or a ray's setup waits on LDS, not on a trip to L2, for the view's constants
  * (which the tile kernel has in SGPRs from its arguments). When the pool's next rays belong to
@@ -221,7 +224,7 @@ __device__ __forceinline__ void stampView(const ArtfHipBatchArgs& a, uint32_t vi
     if (!a.stamps || !lastEnd) return;
     if (lane == 0) {
         const unsigned slot = (blockIdx.x * (kQueueGroup / 64) + (threadIdx.x >> 6)) % ARTF_HIP_STAMP_SLOTS;
-        atomicMax(&a.stamps[1 + view * ARTF_HIP_STAMP_SLOTS + slot], lastEnd);
+        atomicMax(&a.stamps[ARTF_HIP_STAMP_AT(a.viewCount, view, slot)], lastEnd);
     }
     lastEnd = 0;
 }
@@ -264,9 +267,9 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
                     float rgb[3];
                     ray.finish(w, sky, rgb);
                     const ArtfHipView& v = *w.v;
-                    v.out[0][outAt] = rgb[0];
-                    v.out[1][outAt] = rgb[1];
-                    v.out[2][outAt] = rgb[2];
+                    v.rays[0][outAt] = rgb[0];
+                    v.rays[1][outAt] = rgb[1];
+                    v.rays[2][outAt] = rgb[2];
                 }
                 if (a.stamps) lastEnd = wall_clock64();
                 pending = false;
@@ -316,19 +319,18 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
             if (mine != kNoRay) {
                 const ArtfHipView& v = *w.v;
                 const ArtfHoleFrame& f = v.frame;
-                const uint32_t local = (mine >> 6) - (viewBegin >> 6), perSub = v.tilesX * v.tilesY;
-                const uint32_t sub = local / perSub, rest = local - sub * perSub;
-                const uint32_t ty = rest / v.tilesX, tx = rest - ty * v.tilesX;
-                const int px = (int)(tx * 8u + (mine & 7u)), py = (int)(ty * 8u + ((mine >> 3) & 7u));
                 const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
-                if (px < width && py < height) { /* (a tile's places beyond the view's edges stay idle) */
+                const PixelTile tile = pixelTileOf(side);
+                const uint32_t local = (mine >> 6) - (viewBegin >> 6), ty = local / v.tilesX, tx = local - ty * v.tilesX;
+                const TileRay t = tileRayOf((int)(mine & 63u), side, tile, (int)tx * tile.w, (int)ty * tile.h);
+                if (t.traced && t.px < width && t.py < height) { /* (a tile's places beyond the view's edges stay idle) */
                     float dx, dy;
-                    subPixel(v.jitter, px, py, (int)sub % side, (int)sub / side, side, dx, dy);
-                    time = rayMoment(f, (int)sub, side);
+                    subPixel(v.jitter, t.px, t.py, t.sx, t.sy, side, dx, dy);
+                    time = rayMoment(f, t.sub, side);
                     typename Lane::Aim nx, ny;
-                    aimOf(f, px, py, dx, dy, nx, ny);
+                    aimOf(f, t.px, t.py, dx, dy, nx, ny);
                     ray.begin(w, nx, ny, InRangeDrag);
-                    outAt = (side == 1 ? 0u : sub * (uint32_t)width * (uint32_t)height) + (uint32_t)py * (uint32_t)width + (uint32_t)px;
+                    outAt = (side == 1 ? 0u : (uint32_t)t.sub * (uint32_t)width * (uint32_t)height) + (uint32_t)t.py * (uint32_t)width + (uint32_t)t.px;
                     steps = 0, sky = false, active = true;
                 }
             }
@@ -364,13 +366,14 @@ __global__ __launch_bounds__(kQueueGroup) __attribute__((amdgpu_waves_per_eu(wav
 }
 
 /* The batch with no queue (schedule.workgroups = ARTF_HIP_WORKGROUP_A_TILE): a workgroup of one wave for
- * each 8x8 tile of rays, as the frame's tile kernel has, and the hardware hands the workgroups to compute
- * units as their slots free up. A tile is one view's, so the view's frame, ray law and tables are wave-uniform loads (scalar, from
- * the restrict arguments), and each ray is traced by the tile kernel's own code (tracePixelRay) at its
- * registers and waves a SIMD: no pool, queue, LDS copy of the view or refill. It traces what the queued
- * kernel does with refill 64 (new rays only once a wave's 64 lanes are idle), the same tiles the same
- * way, bit for bit; the tuner weighs the two. Workgroup b goes to XCD b % 8, and each XCD takes four
- * neighbouring tiles of every 32, as in traceKernel. */
+ * each tile, as the frame's tile kernel has, and the hardware hands the workgroups to compute units as
+ * their slots free up. A tile is one view's, so the view's frame, ray law and tables are wave-uniform
+ * loads (scalar, from the restrict arguments), each ray is traced by the tile kernel's own code
+ * (tracePixelRay) at its registers and waves a SIMD, and a pixel's rays are summed in the wave and its
+ * mean written to the view's HDR (writeTile): no pool, queue, LDS copy of the view, refill, planes of
+ * rays or resolve. It traces what the queued kernel does with refill 64 (new rays only once a wave's 64
+ * lanes are idle), the same tiles the same way, bit for bit; the tuner weighs the two. Workgroup b goes
+ * to XCD b % 8, and each XCD takes four neighbouring tiles of every 32, as in traceKernel. */
 template <int Precision, bool InRangeDrag>
 __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(Precision)))) void viewTilesKernel(
     const ArtfHipView* __restrict__ views, const uint32_t* __restrict__ firstTile, ArtfHipBatchArgs a) {
@@ -381,13 +384,13 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
     const ArtfHipView& v = views[view];
     const ArtfHoleFrame& f = v.frame;
     const int lane = (int)threadIdx.x;
-    /* the view's tiles sub-ray by sub-ray, rows of tiles top to bottom; the lane's ray in the 8x8 tile */
-    const uint32_t local = tile - firstTile[view], perSub = v.tilesX * v.tilesY;
-    const uint32_t sub = local / perSub, rest = local - sub * perSub;
-    const uint32_t ty = rest / v.tilesX, tx = rest - ty * v.tilesX;
-    const int px = (int)(tx * 8u) + (lane & 7), py = (int)(ty * 8u) + (lane >> 3);
     const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
-    const bool valid = px < width && py < height;
+    /* the view's rows of tiles top to bottom; the lane's ray in its tile, as the frame's tile kernel has it */
+    const PixelTile shape = pixelTileOf(side);
+    const uint32_t local = tile - firstTile[view], ty = local / v.tilesX, tx = local - ty * v.tilesX;
+    const int x0 = (int)tx * shape.w, y0 = (int)ty * shape.h;
+    const TileRay t = tileRayOf(lane, side, shape, x0, y0);
+    const bool valid = t.traced && t.px < width && t.py < height;
 
     TraceLaw law;
     law.maxSteps = v.maxSteps;
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
     tables.fluid = a.fluids ? a.fluids + (size_t)view * ((size_t)ARTF_FLUID_WIDTH * ARTF_FLUID_HEIGHT) : a.map;
 
     RayResult ray = {0.f, 0.f, 0.f, 0};
-    if (valid) ray = tracePixelRay<Precision>(f, v.frame64, law, tables, v.jitter, px, py, (int)(sub % (uint32_t)side), (int)(sub / (uint32_t)side), (int)sub, side);
+    if (valid) ray = tracePixelRay<Precision>(f, v.frame64, law, tables, v.jitter, t.px, t.py, t.sx, t.sy, t.sub, side);
     /* the wave's steps, and its lane-slots as the queued kernel counts them (64 lanes a step the wave takes) */
     if (a.counters) {
         const uint32_t sum = waveSum((uint32_t)ray.steps), most = waveMax((uint32_t)ray.steps);
@@ -412,12 +415,8 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
             atomicAdd(&a.counters[ARTF_HIP_COUNTER_SLOTS + at], (unsigned long long)most * 64ull);
         }
     }
-    if (a.stamps && lane == 0) atomicMax(&a.stamps[1 + view * ARTF_HIP_STAMP_SLOTS + blockIdx.x % ARTF_HIP_STAMP_SLOTS], (unsigned long long)wall_clock64());
-    if (!valid) return;
-    const uint32_t outAt = (side == 1 ? 0u : sub * (uint32_t)width * (uint32_t)height) + (uint32_t)py * (uint32_t)width + (uint32_t)px;
-    v.out[0][outAt] = ray.r;
-    v.out[1][outAt] = ray.g;
-    v.out[2][outAt] = ray.b;
+    if (a.stamps && lane == 0) atomicMax(&a.stamps[ARTF_HIP_STAMP_AT(a.viewCount, view, blockIdx.x % ARTF_HIP_STAMP_SLOTS)], (unsigned long long)wall_clock64());
+    writeTile(ray, valid, lane, side, shape, x0, y0, t.px, t.py, width, height, v.hdr, f.image[3]);
 }
 
 /* The batch's kernel, with the drag's quotient in range where every view's spin allows it: the queued

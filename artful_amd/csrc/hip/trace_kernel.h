@@ -98,13 +98,79 @@ __device__ __forceinline__ void countSteps(unsigned long long* counters, int ste
 }
 
 /* The pixels of a wave: side^2 lanes a pixel, 64 / side^2 pixels (as a tileW x tileH block; with 3, 5,
- * 6 or 7 rays a side some lanes are left over and trace nothing). */
+ * 6 or 7 rays a side some lanes are left over and trace nothing): ARTF_HIP_TILE_W / _H. */
 struct PixelTile {
     int w, h;
 };
 
-__host__ __device__ constexpr PixelTile pixelTileOf(int side) {
-    return side == 1 ? PixelTile{8, 8} : side == 2 ? PixelTile{4, 4} : side == 3 ? PixelTile{7, 1} : side == 4 ? PixelTile{2, 2} : side == 5 ? PixelTile{2, 1} : PixelTile{1, 1};
+__host__ __device__ constexpr PixelTile pixelTileOf(int side) { return PixelTile{ARTF_HIP_TILE_W(side), ARTF_HIP_TILE_H(side)}; }
+
+/* A lane's ray in its wave's tile: the pixel (px, py) from the tile's corner (x0, y0), and its sub-ray
+ * (sx, sy), sub = sy * side + sx; false for the lanes left over. */
+struct TileRay {
+    int px, py, sx, sy, sub;
+    bool traced;
+};
+
+__device__ __forceinline__ TileRay tileRayOf(int lane, int side, PixelTile tile, int x0, int y0) {
+    const int count = side * side;
+    const int pixel = smallQuotient(lane, count), sub = lane - pixel * count; /* the lane's pixel in the tile, and its ray */
+    const int pixelY = smallQuotient(pixel, tile.w), pixelX = pixel - pixelY * tile.w;
+    const int sy = smallQuotient(sub, side);
+    return TileRay{x0 + pixelX, y0 + pixelY, sub - sy * side, sy, sub, pixel < tile.w * tile.h};
+}
+
+/* A wave's rays to its pixels: at one ray a pixel each lane writes its own; at side^2 the rays go through
+ * LDS, then one lane a pixel and colour adds its pixel's rays in the order lensTile adds them, r = r + t
+ * ray by ray from 0 (sy, then sx), and scales by the share (lens.inc:364-380): 3 x pixels lanes each with
+ * a chain of `count` additions, where every lane walking all three colours' chains by cross-lane reads
+ * made the wave wait on each read. Pixels at or past (width, rowEnd) are not written. Called by every
+ * lane of the wave (a one-wave workgroup). */
+__device__ __forceinline__ void writeTile(const RayResult& ray, bool valid, int lane, int side, PixelTile tile, int x0, int y0, int px, int py,
+                                          int width, int rowEnd, float* const* out, float share) {
+    const int count = side * side;
+    if (count == 1) {
+        if (!valid) return;
+        const size_t at = (size_t)py * width + px;
+        out[0][at] = ray.r;
+        out[1][at] = ray.g;
+        out[2][at] = ray.b;
+        return;
+    }
+    __shared__ __align__(16) float light[3][kTraceGroup];
+    light[0][lane] = ray.r;
+    light[1][lane] = ray.g;
+    light[2][lane] = ray.b;
+    __syncthreads();
+    const int pixels = tile.w * tile.h;
+    if (lane >= 3 * pixels) return;
+    const int colour = smallQuotient(lane, pixels), mine = lane - colour * pixels;
+    const int mineY = smallQuotient(mine, tile.w);
+    const int qx = x0 + mine - mineY * tile.w, qy = y0 + mineY;
+    if (qx >= width || qy >= rowEnd) return;
+    float* plane = colour == 0 ? out[0] : colour == 1 ? out[1] : out[2];
+    const float* rays = &light[colour][mine * count];
+    float sum = 0.f;
+    if ((count & 3) == 0) {
+        /* mine * count * 4 bytes from a 16-byte aligned array: whole float4s, four of them read before their
+         * sixteen additions where there are sixteen rays left */
+        const float4* quads = (const float4*)rays;
+        int k = 0;
+        for (; k + 4 <= count / 4; k += 4) {
+            const float4 t0 = quads[k], t1 = quads[k + 1], t2 = quads[k + 2], t3 = quads[k + 3];
+            sum = sum + t0.x, sum = sum + t0.y, sum = sum + t0.z, sum = sum + t0.w;
+            sum = sum + t1.x, sum = sum + t1.y, sum = sum + t1.z, sum = sum + t1.w;
+            sum = sum + t2.x, sum = sum + t2.y, sum = sum + t2.z, sum = sum + t2.w;
+            sum = sum + t3.x, sum = sum + t3.y, sum = sum + t3.z, sum = sum + t3.w;
+        }
+        for (; k < count / 4; ++k) {
+            const float4 t = quads[k];
+            sum = sum + t.x, sum = sum + t.y, sum = sum + t.z, sum = sum + t.w;
+        }
+    } else {
+        for (int k = 0; k < count; ++k) sum = sum + rays[k];
+    }
+    plane[(size_t)qy * width + qx] = sum * share;
 }
 
 /* Sub-ray (sx, sy) of pixel (px, py), sub = sy * side + sx of side^2, traced in the given precision: its
@@ -140,12 +206,10 @@ template <int Precision, bool InRangeDrag = false>
 __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(traceWavesPerSimd(Precision)))) void traceKernel(ArtfHipTraceArgs a) {
     stampStart(a.startStamp, a.spanStamp);
     const ArtfHoleFrame& f = a.frame;
-    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2], count = side * side;
+    const int width = (int)f.size[0], height = (int)f.size[1], side = (int)f.size[2];
     const int rowEnd = a.rowEnd > 0 ? min(a.rowEnd, height) : height;
     const PixelTile tile = pixelTileOf(side);
     const int lane = (int)threadIdx.x;
-    const int pixel = smallQuotient(lane, count), sub = lane - pixel * count; /* the lane's pixel in the tile, and its ray */
-    const int pixelY = smallQuotient(pixel, tile.w), pixelX = pixel - pixelY * tile.w;
     /* The hardware deals workgroup b = y * gridDim.x + x to XCD b % 8, and gridDim.x is a multiple of 32:
      * the x-th workgroup of a row is slot x / 8 of XCD x % 8, and each four slots of an XCD take four tiles
      * side by side. Their pixels then share the 64-byte pieces of each HDR row in one XCD's L2, which
@@ -154,10 +218,9 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
     const unsigned xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3;
     const int tileX = (int)(((slot >> 2) * 8u + xcd) * 4u + (slot & 3u)), tileY = (int)blockIdx.y;
     if (tileX * tile.w >= width) return; /* the row's tiles rounded up to 32: the whole wave leaves */
-    const int px = tileX * tile.w + pixelX;
-    const int py = a.rowBegin + tileY * tile.h + pixelY;
-    const int sy = smallQuotient(sub, side), sx = sub - sy * side;
-    const bool valid = pixel < tile.w * tile.h && px < width && py < rowEnd;
+    const int x0 = tileX * tile.w, y0 = a.rowBegin + tileY * tile.h;
+    const TileRay t = tileRayOf(lane, side, tile, x0, y0);
+    const bool valid = t.traced && t.px < width && t.py < rowEnd;
 
     TraceLaw law;
     law.maxSteps = a.maxSteps > 0 ? a.maxSteps : 192;
@@ -167,54 +230,9 @@ __global__ __launch_bounds__(kTraceGroup) __attribute__((amdgpu_waves_per_eu(tra
     law.inRangeDrag = InRangeDrag;
 
     RayResult ray = {0.f, 0.f, 0.f, 0};
-    if (valid) ray = tracePixelRay<Precision>(f, a.frame64, law, tablesOf(a.map, a.fluid), a.jitter, px, py, sx, sy, sub, side);
+    if (valid) ray = tracePixelRay<Precision>(f, a.frame64, law, tablesOf(a.map, a.fluid), a.jitter, t.px, t.py, t.sx, t.sy, t.sub, side);
     countSteps(a.counters, ray.steps, valid);
-    if (count == 1) {
-        if (!valid) return;
-        const size_t at = (size_t)py * width + px;
-        a.out[0][at] = ray.r;
-        a.out[1][at] = ray.g;
-        a.out[2][at] = ray.b;
-        return;
-    }
-    /* The pixels' rays through LDS, then one lane a pixel and colour adds its pixel's rays in the order
-     * lensTile adds them, r = r + t ray by ray from 0 (sy, then sx), and scales by the share: 3 x pixels
-     * lanes each with a chain of `count` additions, where every lane walking all three colours' chains by
-     * cross-lane reads made the wave wait on each read. */
-    __shared__ __align__(16) float light[3][kTraceGroup];
-    light[0][lane] = ray.r;
-    light[1][lane] = ray.g;
-    light[2][lane] = ray.b;
-    __syncthreads();
-    const int pixels = tile.w * tile.h;
-    if (lane >= 3 * pixels) return;
-    const int colour = smallQuotient(lane, pixels), mine = lane - colour * pixels;
-    const int mineY = smallQuotient(mine, tile.w);
-    const int qx = tileX * tile.w + mine - mineY * tile.w, qy = a.rowBegin + tileY * tile.h + mineY;
-    if (qx >= width || qy >= rowEnd) return;
-    float* plane = colour == 0 ? a.out[0] : colour == 1 ? a.out[1] : a.out[2];
-    const float* rays = &light[colour][mine * count];
-    float sum = 0.f;
-    if ((count & 3) == 0) {
-        /* mine * count * 4 bytes from a 16-byte aligned array: whole float4s, four of them read before their
-         * sixteen additions where there are sixteen rays left */
-        const float4* quads = (const float4*)rays;
-        int k = 0;
-        for (; k + 4 <= count / 4; k += 4) {
-            const float4 t0 = quads[k], t1 = quads[k + 1], t2 = quads[k + 2], t3 = quads[k + 3];
-            sum = sum + t0.x, sum = sum + t0.y, sum = sum + t0.z, sum = sum + t0.w;
-            sum = sum + t1.x, sum = sum + t1.y, sum = sum + t1.z, sum = sum + t1.w;
-            sum = sum + t2.x, sum = sum + t2.y, sum = sum + t2.z, sum = sum + t2.w;
-            sum = sum + t3.x, sum = sum + t3.y, sum = sum + t3.z, sum = sum + t3.w;
-        }
-        for (; k < count / 4; ++k) {
-            const float4 t = quads[k];
-            sum = sum + t.x, sum = sum + t.y, sum = sum + t.z, sum = sum + t.w;
-        }
-    } else {
-        for (int k = 0; k < count; ++k) sum = sum + rays[k];
-    }
-    plane[(size_t)qy * width + qx] = sum * f.image[3];
+    writeTile(ray, valid, lane, side, tile, x0, y0, t.px, t.py, width, rowEnd, a.out, f.image[3]);
 }
 
 inline hipError_t launchTraceWith(void (*kernel)(ArtfHipTraceArgs), const ArtfHipTraceArgs& a, hipStream_t stream) {

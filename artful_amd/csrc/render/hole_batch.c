@@ -4,9 +4,10 @@
  * colours, sky, dust) for all views; per view its clamped description, FP32 and FP64 frames, ring
  * tables and (painted disc) fluid
  * with the view table and the views' tile offsets. Each draw replays one HIP graph: the workspace
- * reset kernel, the trace over all views (a workgroup a tile, or persistent and queued), and the
- * resolve of each view with several rays a pixel. The tuner times candidate schedules on the batch
- * itself and keeps the fastest. */
+ * reset kernel, the trace over all views (a workgroup a tile, which writes each pixel's mean, or
+ * persistent and queued, which writes each ray's light), and after the queued kernel the resolve of each
+ * view with several rays a pixel. The tuner times candidate schedules on the batch itself and keeps the
+ * fastest. */
 #include <stdio.h>
 #include <string.h>
 
@@ -22,7 +23,7 @@ typedef struct BatchView {
     ArtfHoleFrame frame;
     int width, height, side;
     float* hdr[3];  /* device, width * height */
-    float* rays[3]; /* device, width * height * side^2 when side > 1 */
+    float* rays[3]; /* device, width * height * side^2 when side > 1, once a queued schedule is captured */
 } BatchView;
 
 struct ArtfHoleBatch {
@@ -37,6 +38,8 @@ struct ArtfHoleBatch {
     float* rings;             /* device, count * RING_FLOATS */
     float* fluids;            /* device, count * FLUID_FLOATS, painted disc only */
     ArtfHipView* deviceViews; /* device, count */
+    ArtfHipView* hostViews;   /* the same, as uploaded */
+    bool rayPlanes;           /* the views with several rays a pixel have their planes of rays */
     uint32_t* firstTile;      /* device, count + 1 */
     void* workspace;          /* device: two sets of queues, counters, stamps */
     uint32_t* queues[2];      /* in the workspace: a set of queues for each stream */
@@ -70,6 +73,7 @@ void artfHoleBatchFree(ArtfHoleBatch* b) {
     artfFree(b->views);
     artfDeviceMapFree(b->map), artfHipFree(b->rings), artfHipFree(b->fluids);
     artfHipFree(b->deviceViews), artfHipFree(b->firstTile), artfHipFree(b->workspace);
+    artfFree(b->hostViews);
     artfFree(b->countersHost), artfFree(b->stampsHost);
     if (b->side) artfHipStreamSync(b->side);
     for (int i = 0; i < 2; ++i) artfHipEventDestroy(b->events[i]), artfHipEventDestroy(b->order[i]);
@@ -119,10 +123,10 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
             artfLog("artful: view %d's size %dx%d is out of range", v, w, h);
             return NULL;
         }
-        tiles += (uint64_t)((w + 7) / 8) * ((h + 7) / 8) * d.samples * d.samples;
+        tiles += (uint64_t)((w + ARTF_HIP_TILE_W(d.samples) - 1) / ARTF_HIP_TILE_W(d.samples)) * ((h + ARTF_HIP_TILE_H(d.samples) - 1) / ARTF_HIP_TILE_H(d.samples));
     }
     if (tiles > (1u << 25)) {
-        artfLog("artful: a batch holds at most 2^25 tiles of 8x8 rays");
+        artfLog("artful: a batch holds at most 2^25 tiles of 64 rays");
         return NULL;
     }
     if (device < 0 || device >= artfHipDeviceCount() || artfHipSetDevice(device)) {
@@ -173,22 +177,23 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
         }
         artfHoleFrameOf(&d, &given->camera, fluidTime, bv->width, bv->height, 1, &bv->frame, NULL);
         const size_t pixels = (size_t)bv->width * bv->height;
-        ready = ready && allocPlanes(bv->hdr, pixels) && (bv->side == 1 || allocPlanes(bv->rays, pixels * bv->side * bv->side));
+        ready = ready && allocPlanes(bv->hdr, pixels);
         if (!ready) break;
         ArtfHipView* h = &hv[v];
         h->frame = bv->frame;
         artfHoleFrame64Of(&d, &given->camera, bv->width, bv->height, &h->frame64);
-        for (int c = 0; c < 3; ++c) h->out[c] = bv->side > 1 ? bv->rays[c] : bv->hdr[c];
+        for (int c = 0; c < 3; ++c) h->hdr[c] = bv->hdr[c], h->rays[c] = bv->side > 1 ? NULL : bv->hdr[c];
         h->maxSteps = t.maxSteps, h->escapeRadius = t.escapeRadius, h->disc = t.disc, h->emissivity = t.emissivity, h->jitter = t.jitter;
-        h->tilesX = (uint32_t)(bv->width + 7) / 8, h->tilesY = (uint32_t)(bv->height + 7) / 8;
-        h->rays = (uint32_t)(pixels * bv->side * bv->side);
+        const int tileW = ARTF_HIP_TILE_W(bv->side), tileH = ARTF_HIP_TILE_H(bv->side);
+        h->tilesX = (uint32_t)((bv->width + tileW - 1) / tileW), h->tilesY = (uint32_t)((bv->height + tileH - 1) / tileH);
+        h->rayCount = (uint32_t)(pixels * bv->side * bv->side);
         firstTile[v] = at;
-        at += h->tilesX * h->tilesY * (uint32_t)(bv->side * bv->side);
-        b->rays += h->rays;
+        at += h->tilesX * h->tilesY;
+        b->rays += h->rayCount;
     }
     firstTile[count] = at;
 
-    const size_t slots = 2 * ARTF_HIP_COUNTER_SLOTS, stamps = 1 + (size_t)count * ARTF_HIP_STAMP_SLOTS;
+    const size_t slots = 2 * ARTF_HIP_COUNTER_SLOTS, stamps = ARTF_HIP_STAMP_WORDS(count);
     const size_t queueBytes = (size_t)ARTF_HIP_QUEUES * ARTF_HIP_QUEUE_STRIDE * 4;
     const size_t workBytes = 2 * queueBytes + slots * 8 + stamps * 8;
     if (ready) {
@@ -200,7 +205,10 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
                 !artfHipUpload(b->firstTile, firstTile, (size_t)(count + 1) * sizeof(uint32_t), b->stream) && !artfHipStreamSync(b->stream);
     }
     artfHipStreamSync(b->stream); /* the uploads have read the host copies */
-    artfFree(rings), artfFree(hv), artfFree(firstTile);
+    artfFree(rings), artfFree(firstTile);
+    b->hostViews = hv;
+    b->rayPlanes = true;
+    for (int v = 0; v < count; ++v) b->rayPlanes = b->rayPlanes && b->views[v].side == 1;
     if (!ready) return fail(b, "set up its device memory");
 
     char* w = (char*)b->workspace;
@@ -220,11 +228,34 @@ ArtfHoleBatch* artfHoleBatchNew(const ArtfHoleView* views, int count, const Artf
     return b;
 }
 
-/* The graph of one draw: the reset, the trace over all views, each view's resolve. In slices of
- * launchTiles tiles, the slices take turns on two streams, each with its own set of queues (cleared
- * before each of its slices, for the queued kernel): a slice's last waves, as its launch drains, run
- * beside the next slice's first, instead of the device idling between launches. */
+/* The planes of rays the queued kernel writes for the views with several rays a pixel (it cannot sum a
+ * pixel's rays in a wave whose lanes it refills), made the first time a queued schedule is captured, and
+ * the view table uploaded again with them. */
+static bool makeRayPlanes(ArtfHoleBatch* b) {
+    if (b->rayPlanes) return true;
+    for (int v = 0; v < b->count; ++v) {
+        BatchView* bv = &b->views[v];
+        if (bv->side == 1 || bv->rays[0]) continue;
+        if (!allocPlanes(bv->rays, (size_t)bv->width * bv->height * bv->side * bv->side)) {
+            freePlanes(bv->rays);
+            artfLog("artful: the batch's planes of rays do not fit on the device");
+            return false;
+        }
+        for (int c = 0; c < 3; ++c) b->hostViews[v].rays[c] = bv->rays[c];
+    }
+    if (artfHipUpload(b->deviceViews, b->hostViews, (size_t)b->count * sizeof(ArtfHipView), b->stream) || artfHipStreamSync(b->stream)) return false;
+    b->rayPlanes = true;
+    return true;
+}
+
+/* The graph of one draw: the reset, the trace over all views and, after the queued kernel, the resolve
+ * of each view with several rays a pixel. In slices of launchTiles tiles, the slices take turns on two
+ * streams, each with its own set of queues (cleared before each of its slices, for the queued kernel): a
+ * slice's last waves, as its launch drains, run beside the next slice's first, instead of the device
+ * idling between launches. */
 static bool capture(ArtfHoleBatch* b) {
+    const bool queued = b->schedule.workgroups != ARTF_HOLE_WORKGROUP_A_TILE;
+    if (queued && !makeRayPlanes(b)) return false;
     ArtfHipBatchArgs* a = &b->args;
     a->schedule.workgroups = b->schedule.workgroups, a->schedule.chunkTiles = b->schedule.chunkTiles;
     a->schedule.refill = b->schedule.refill, a->schedule.stepsPerCheck = b->schedule.stepsPerCheck;
@@ -250,7 +281,7 @@ static bool capture(ArtfHoleBatch* b) {
         status = artfHipEventRecord(b->order[1], b->side);
         if (!status) status = artfHipStreamWait(b->stream, b->order[1]);
     }
-    for (int v = 0; !status && v < b->count; ++v)
+    for (int v = 0; queued && !status && v < b->count; ++v) /* the tile form wrote each pixel's mean itself */
         if (b->views[v].side > 1) status = artfHipResolve(&b->views[v].frame, b->views[v].rays, b->views[v].hdr, b->stream);
     void* graph = NULL;
     const int ended = artfHipGraphEnd(b->stream, &graph);
@@ -273,7 +304,7 @@ bool artfHoleBatchDraw(ArtfHoleBatch* b) {
     }
     b->batchMs = artfHipEventMs(b->events[0], b->events[1]);
     if (artfHipDownload(b->countersHost, b->args.counters, 2 * ARTF_HIP_COUNTER_SLOTS * 8, b->stream) ||
-        artfHipDownload(b->stampsHost, b->args.stamps, (1 + (size_t)b->count * ARTF_HIP_STAMP_SLOTS) * 8, b->stream))
+        artfHipDownload(b->stampsHost, b->args.stamps, (size_t)ARTF_HIP_STAMP_WORDS(b->count) * 8, b->stream))
         return false;
     b->drawn = true;
     return true;
@@ -292,7 +323,7 @@ static double stampMs(const ArtfHoleBatch* b, int view) {
     const unsigned long long start = b->stampsHost[0];
     unsigned long long end = 0;
     for (int k = 0; k < ARTF_HIP_STAMP_SLOTS; ++k) {
-        const unsigned long long t = b->stampsHost[1 + (size_t)view * ARTF_HIP_STAMP_SLOTS + k];
+        const unsigned long long t = b->stampsHost[ARTF_HIP_STAMP_AT(b->count, view, k)];
         end = t > end ? t : end;
     }
     if (b->clockHz <= 0 || end < start) return -1.0;

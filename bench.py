@@ -487,9 +487,8 @@ def extended_form(native, renderer=None, scene=None, steps=100, warmup=30):
 
 def huge_frame(native, width=3840, height=2160, side=8, launch_tiles=1 << 20, draws=3):
     """The huge frame: 3840x2160 at 8x8 rays a pixel (530,841,600 rays) as a batch of one view in slices
-    of 2^20 tiles (67 M rays, ~7 ms a launch: watchdog-sized; the batch writes its rays' 6.4 GB of planes
-    and resolves them), beside the renderer's single launch of the tile kernel (each pixel's 64 rays
-    summed in one wave's lanes)."""
+    of 2^20 tiles (67 M rays, ~6 ms a launch: watchdog-sized; each pixel's 64 rays summed in one wave,
+    under the default schedule), beside the renderer's banded launches of the tile kernel."""
     desc, cam = workload_scene(native, samples=side)
     out = {"width": width, "height": height, "rays_per_pixel": side * side, "rays": width * height * side * side,
            "launch_tiles": launch_tiles}

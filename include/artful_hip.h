@@ -103,22 +103,34 @@ int artfHipTrace(const ArtfHipTraceArgs* args, void* stream);
  * as a batch's views and the adjoint write them; hdr: width * height each. */
 int artfHipResolve(const ArtfHoleFrame* frame, float* const rays[3], float* const hdr[3], void* stream);
 
+/* The pixels of one wave's tile at side^2 rays a pixel, the pixel's rays in adjacent lanes: 8x8 pixels at
+ * one ray, 4x4 at 2x2 rays, 7x1 at 3x3, 2x2 at 4x4, 2x1 at 5x5, one pixel at 6x6 to 8x8 (with 3, 5, 6 or
+ * 7 rays a side some of the 64 lanes trace nothing). The frame's tracer and the batches take the same. */
+#define ARTF_HIP_TILE_W(side) ((side) == 1 ? 8 : (side) == 2 ? 4 : (side) == 3 ? 7 : (side) == 4 || (side) == 5 ? 2 : 1)
+#define ARTF_HIP_TILE_H(side) ((side) == 1 ? 8 : (side) == 2 ? 4 : (side) == 4 ? 2 : 1)
+
 /* ---- batches of views (not in the reference, which draws one view a frame): many views of any
- * size, spin, eye and disc, traced by one persistent launch whose waves take 8x8 tiles of rays from a
- * queue and give idle lanes new rays of the view they are on (a wave holds one view's rays at a time).
- * Each view's rays are those artfHipTrace would trace for its frame, bit for bit in FP32. ---- */
+ * size, spin, eye and disc, traced by one launch: a workgroup of one wave for each tile, or persistent
+ * waves that take tiles of rays from a queue and give idle lanes new rays of the view they are on (a
+ * wave holds one view's rays at a time). Each view's rays are those artfHipTrace would trace for its
+ * frame, bit for bit in FP32. ---- */
 
 typedef struct ArtfHipView {
     ArtfHoleFrame frame;     /* artfHoleFrame's FP32 uniforms for this view */
     ArtfHoleFrame64 frame64; /* read by the MIXED and FP64 tracers */
-    float* out[3];           /* device planes, as artfHipTrace's out[] for this view's frame */
+    float* hdr[3];           /* device planes of width * height: each pixel's mean, as artfHipTrace's out[] */
+    /* the queued kernel's output: hdr at one ray a pixel; else device planes of each ray's light (ray k of
+     * pixel p at k * width * height + p) for artfHipResolve, which it cannot sum in a wave whose lanes it
+     * refills; NULL where no queued launch will run */
+    float* rays[3];
     int32_t maxSteps;        /* at least 1 */
     float escapeRadius;      /* 0: where 1/r crosses 0 */
     int32_t disc;            /* ArtfDiscModel */
     float emissivity;        /* the analytic disc's q */
     uint32_t jitter;         /* 0: the stratified grid; else a seed */
-    uint32_t tilesX, tilesY; /* (width + 7) / 8, (height + 7) / 8 */
-    uint32_t rays;           /* width * height * side^2 */
+    /* tiles of ARTF_HIP_TILE_W(side) x ARTF_HIP_TILE_H(side) pixels across and down the view, rounded up */
+    uint32_t tilesX, tilesY;
+    uint32_t rayCount;       /* width * height * side^2 */
 } ArtfHipView;
 
 /* A view's ring tables in ArtfHipBatchArgs.rings: ring speed and heat (ARTF_HOLE_MAP_HEIGHT x 2, as
@@ -126,6 +138,12 @@ typedef struct ArtfHipView {
  * ARTF_HOLE_GLOW_AT). */
 #define ARTF_HIP_RING_FLOATS (ARTF_HOLE_MAP_HEIGHT * 3)
 #define ARTF_HIP_STAMP_SLOTS 16
+/* ArtfHipBatchArgs.stamps: the launch's start at word 0, then a row of slots for each k of
+ * ARTF_HIP_STAMP_SLOTS, the views side by side in it, the rows 512 bytes apart or more: view v's slots
+ * lie in 16 different cache lines, so the atomics of its waves ending together do not all queue on one */
+#define ARTF_HIP_STAMP_ROW(views) (((uint32_t)(views) + 63u) & ~63u)
+#define ARTF_HIP_STAMP_AT(views, view, k) (1u + (uint32_t)(k) * ARTF_HIP_STAMP_ROW(views) + (uint32_t)(view))
+#define ARTF_HIP_STAMP_WORDS(views) (1u + ARTF_HIP_STAMP_SLOTS * ARTF_HIP_STAMP_ROW(views))
 #define ARTF_HIP_QUEUES 8
 #define ARTF_HIP_QUEUE_STRIDE 32
 
@@ -149,7 +167,7 @@ typedef struct ArtfHipSchedule {
 typedef struct ArtfHipBatchArgs {
     const ArtfHipView* views;  /* device, viewCount */
     const uint32_t* firstTile; /* device, viewCount + 1: view v's tiles are [firstTile[v], firstTile[v + 1]), each
-                                * view's tiles sub-ray by sub-ray, rows of tiles top to bottom */
+                                * view's rows of tiles top to bottom; ray k of a tile is lane k of its wave */
     int32_t viewCount;
     /* the tiles this launch traces, [tileBegin, tileEnd) of [0, firstTile[viewCount]): a huge batch is
      * traced by several launches of a slice each (watchdog-sized), the queues cleared between them */
@@ -169,8 +187,8 @@ typedef struct ArtfHipBatchArgs {
     /* device, 2 * ARTF_HIP_COUNTER_SLOTS, as artfHipTrace's counters (lane-slots: every lane of every
      * stepping wave, idle ones included); NULL: not counted */
     unsigned long long* counters;
-    /* device, 1 + viewCount * ARTF_HIP_STAMP_SLOTS words, or NULL: the launch's start (stamps[0]), then
-     * per view v the latest moment a wave finished rays of it, in slots 1 + v * ARTF_HIP_STAMP_SLOTS + k
+    /* device, ARTF_HIP_STAMP_WORDS(viewCount) words, or NULL: the launch's start (stamps[0]), then per
+     * view v the latest moment a wave finished rays of it, in slots ARTF_HIP_STAMP_AT(viewCount, v, k)
      * (the view's last ray ended at the largest of its slots), in the device's wall clock
      * (artfHipWallClockHz). Spread over slots, as the counters are. */
     unsigned long long* stamps;

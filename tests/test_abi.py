@@ -31,7 +31,7 @@ def test_struct_sizes():
     assert C.sizeof(abi.HoleFrame) == 368
     assert C.sizeof(abi.HoleFrame64) == 352
     assert C.sizeof(abi.HipTraceArgs) == 816 and C.sizeof(abi.HipFrameArgs) == 904
-    assert C.sizeof(abi.HipView) == 776 and C.sizeof(abi.HipBatchArgs) == 104 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
+    assert C.sizeof(abi.HipView) == 800 and C.sizeof(abi.HipBatchArgs) == 104 and C.sizeof(abi.HoleBatchStats) == 64 and C.sizeof(abi.HoleSchedule) == 20
     assert abi.MAP_FLOATS == 2887424
 
 
