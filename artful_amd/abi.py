@@ -47,7 +47,7 @@ class HoleFrame64(C.Structure):
 
 class HoleTracing(C.Structure):
     _fields_ = [("precision", C.c_int), ("disc", C.c_int), ("emissivity", C.c_float), ("maxSteps", C.c_int),
-                ("escapeRadius", C.c_float), ("jitter", C.c_uint32), ("bloomLevels", C.c_int), ("reserved", C.c_int)]
+                ("escapeRadius", C.c_float), ("jitter", C.c_uint32), ("bloomLevels", C.c_int), ("uncounted", C.c_int)]
 
 
 class HoleStats(C.Structure):

@@ -191,7 +191,7 @@ int artfHipToneMap(const ArtfHoleFrame* f, float* const hdr[3], float* const blo
 }
 
 int artfHipDrawFrame(const ArtfHipFrameArgs* a, void* stream) {
-    if (!a || !traceFits(&a->trace) || !pyramidFits(&a->trace.frame) || !a->bgra || (a->report && !a->trace.counters)) return (int)hipErrorInvalidValue;
+    if (!a || !traceFits(&a->trace) || !pyramidFits(&a->trace.frame) || !a->bgra) return (int)hipErrorInvalidValue;
     const ArtfHoleFrame* f = &a->trace.frame;
     if (a->trace.rowBegin != 0 || (a->trace.rowEnd != 0 && a->trace.rowEnd < (int)f->size[1])) return (int)hipErrorInvalidValue; /* whole frames */
     for (int c = 0; c < 3; ++c)

@@ -57,11 +57,12 @@ struct Level {
 
 /* The trace's step counters (2 x ARTF_HIP_COUNTER_SLOTS words, written before this kernel on its
  * stream) summed into settled[0..1] in host memory, with system-scope stores the host sees once the
- * stream is synchronised, and cleared for the next frame's trace. The first workgroup (one wave) of the
- * kernel that follows the trace calls it, before anything can return. */
+ * stream is synchronised, and cleared for the next frame's trace; zeros where the trace did not count.
+ * The first workgroup (one wave) of the kernel that follows the trace calls it, before anything can
+ * return. */
 struct Settle {
-    unsigned long long* counters;
-    unsigned long long* settled; /* NULL: nothing to do */
+    unsigned long long* counters; /* NULL: not counted */
+    unsigned long long* settled;  /* NULL: nothing to do */
 };
 
 __device__ void settleCounters(const Settle& z) {
@@ -71,7 +72,7 @@ __device__ void settleCounters(const Settle& z) {
     const int t = (int)threadIdx.x;
     for (int half = 0; half < 2; ++half) {
         unsigned long long sum = 0;
-        for (int i = t; i < ARTF_HIP_COUNTER_SLOTS; i += kPost) {
+        for (int i = t; counters && i < ARTF_HIP_COUNTER_SLOTS; i += kPost) {
             sum += counters[half * ARTF_HIP_COUNTER_SLOTS + i];
             counters[half * ARTF_HIP_COUNTER_SLOTS + i] = 0ull;
         }

@@ -284,7 +284,7 @@ static bool hipDraw(ArtfRenderer* r, ArtfScene* scene) {
         if (!done && !ok(h, artfHipStreamWait(h->traces[set], h->done[set]), "frame order")) return false;
     }
     for (int c = 0; c < 3; ++c) a->out[c] = fa.hdr[c] = h->hdr[set][c], fa.bloom[c] = h->bloom[set][c];
-    a->counters = h->counters[set];
+    a->counters = h->tracing.uncounted ? NULL : h->counters[set];
     a->precision = h->tracing.precision;
     a->maxSteps = h->tracing.maxSteps;
     a->escapeRadius = h->tracing.escapeRadius;

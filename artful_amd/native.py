@@ -250,12 +250,12 @@ class Renderer:
         return self._L.artfRendererDescription(self.ptr).decode()
 
     def set_tracing(self, precision="fp32", disc="painted", emissivity=3.0, max_steps=0, escape_radius=0.0,
-                    jitter=0, bloom_levels=0) -> None:
+                    jitter=0, bloom_levels=0, count_steps=True) -> None:
         t = self._L.artfHoleTracingDefaults()
         t.precision = abi.PRECISION[precision] if isinstance(precision, str) else precision
         t.disc = abi.DISC[disc] if isinstance(disc, str) else disc
         t.emissivity, t.maxSteps, t.escapeRadius = emissivity, max_steps, escape_radius
-        t.jitter, t.bloomLevels = jitter, bloom_levels
+        t.jitter, t.bloomLevels, t.uncounted = jitter, bloom_levels, 0 if count_steps else 1
         self._L.artfRendererSetHoleTracing(self.ptr, C.byref(t))
 
     def resize(self, width: int, height: int) -> None:

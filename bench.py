@@ -60,8 +60,10 @@ def settle_frames(renderer, scene, seconds):
 def time_frames(native, renderer, scene, precision, steps, warmup, settle_s=0.0):
     """K frames back to back (artfRendererDraw queues a frame; the last one is waited for by the stats
     read after the loop, inside the timed region), after `settle_s` seconds of untimed frames and then W
-    untimed warm-up frames; then frames one at a time for each stage's GPU time and the step counts."""
-    renderer.set_tracing(precision)
+    untimed warm-up frames, their steps not counted (ArtfHoleTracing.uncounted: the trace skips the
+    counting's wave reductions and atomics, which are the stats', not the image's); then frames one at a
+    time, counted, for each stage's GPU time and the step counts (the same rays: the same steps)."""
+    renderer.set_tracing(precision, count_steps=False)
     settled = settle_frames(renderer, scene, settle_s) if settle_s > 0 else 0
     for _ in range(warmup):
         if not renderer.draw(scene):
@@ -76,6 +78,7 @@ def time_frames(native, renderer, scene, precision, steps, warmup, settle_s=0.0)
     # the same frames on the GPU's own clock: the first one's trace start to the last one's tone-map end
     assert st.frames == steps, (st.frames, steps)
     gpu_span = st.spanMs
+    renderer.set_tracing(precision)
     trace, post, slots, nsteps = [], [], 0, 0
     staged = max(3, min(steps, 10))
     for _ in range(staged):
@@ -93,8 +96,9 @@ def time_frames(native, renderer, scene, precision, steps, warmup, settle_s=0.0)
 def kernel_ms(native, scene, precision, launches=20, warmup=100):
     """The trace kernel alone on the workload's frame: artfHipTrace launched back to back on a stream of
     its own, bracketed by two HIP events on that stream; milliseconds a launch. The kernel writes the
-    HDR planes (each pixel's rays summed in its lanes) and counts its steps, as in a frame. The untimed launches first (about 0.1 s) bring the
-    GPU's clock up from where an idle spell left it."""
+    HDR planes (each pixel's rays summed in its lanes) and does not count its steps, as in the timed
+    frames. The untimed launches first (about 0.1 s) bring the GPU's clock up from where an idle spell
+    left it."""
     import ctypes as C
     from artful_amd import abi
     L = native.lib()
@@ -104,8 +108,7 @@ def kernel_ms(native, scene, precision, launches=20, warmup=100):
     a.frame64 = scene.frame64(WIDTH, HEIGHT)
     a.precision = abi.PRECISION[precision]
     host = [np.ascontiguousarray(scene.map()), np.ascontiguousarray(scene.fluid()[0])]
-    counters = 2 * 1024 * 8  # 2 x ARTF_HIP_COUNTER_SLOTS words
-    held = [L.artfHipMalloc(h.nbytes) for h in host] + [L.artfHipMalloc(WIDTH * HEIGHT * 4) for _ in range(3)] + [L.artfHipMalloc(counters)]
+    held = [L.artfHipMalloc(h.nbytes) for h in host] + [L.artfHipMalloc(WIDTH * HEIGHT * 4) for _ in range(3)]
     stream, start, stop = L.artfHipStreamCreate(), L.artfHipEventCreate(), L.artfHipEventCreate()
     try:
         if not all(held) or not (stream and start and stop):
@@ -116,9 +119,7 @@ def kernel_ms(native, scene, precision, launches=20, warmup=100):
         a.map, a.fluid = held[0], held[1]
         for c in range(3):
             a.out[c] = held[2 + c]
-        a.counters = held[5]
-        if L.artfHipMemset(a.counters, 0, counters, stream):
-            raise RuntimeError("memset failed")
+        a.counters = None
         for _ in range(warmup):
             if L.artfHipTrace(C.byref(a), stream):
                 raise RuntimeError("trace failed")

@@ -285,8 +285,8 @@ int artfHipToneMap(const ArtfHoleFrame* frame, float* const hdr[3], float* const
 #define ARTF_HIP_REPORT_WORDS 7
 typedef struct ArtfHipFrameArgs {
     /* out[]: the HDR planes, hdr[] below; whole frames (rowBegin 0, rowEnd 0 or the height); startStamp:
-     * set by artfHipDrawFrame (spanStamp is kept as given). counters: required with a report, and zero
-     * before the first frame (each frame leaves them zero). */
+     * set by artfHipDrawFrame (spanStamp is kept as given). counters: zero before the first frame (each
+     * frame leaves them zero), or NULL: not counted, the report's steps and lane-slots then 0. */
     ArtfHipTraceArgs trace;
     float* hdr[3];   /* device, width * height each */
     float* bloom[3]; /* device, the pyramid as artfHipBloom lays it out */

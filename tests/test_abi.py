@@ -143,7 +143,8 @@ def test_trace_rejects_bad_arguments(native):
 
 def test_frame_rejects_bad_arguments(native):
     """artfHipDrawFrame checks what the four calls it stands for check, and its own: the output planes,
-    the HDR as the trace's output with one ray a pixel, counters under a report, whole frames only."""
+    the HDR as the trace's output with one ray a pixel, a handoff event with a second stream, whole frames
+    only."""
     L = native.lib()
     INVALID = 1  # hipErrorInvalidValue: refused on the host, nothing launched
     assert L.artfHipDrawFrame(None, None) == INVALID
@@ -158,9 +159,6 @@ def test_frame_rejects_bad_arguments(native):
     for c in range(3):
         a.trace.out[c], a.hdr[c], a.bloom[c] = 64 * (c + 1), 64 * (c + 4), 64 * (c + 7)
     a.bgra = 64
-    a.report = 64  # a report without counters
-    assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
-    a.report = None
     a.postStream = 64  # a second stream without the handoff event
     assert L.artfHipDrawFrame(C.byref(a), None) == INVALID
     a.postStream = None
