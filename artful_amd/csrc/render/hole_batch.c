@@ -264,6 +264,7 @@ static bool capture(ArtfHoleBatch* b) {
     void* streams[2] = {b->stream, b->side};
     if (artfHipGraphBegin(b->stream)) return false;
     ArtfHipBatchArgs x = *a;
+    if (b->tracing.uncounted) x.counters = NULL; /* the steps not counted: the stats' steps and lane-slots read 0 */
     x.queue = b->queues[0], x.tileBegin = 0, x.tileEnd = slice < b->tiles ? slice : b->tiles;
     int status = artfHipBatchReset(&x, b->stream); /* queues of the first set, counters, stamps */
     if (!status && split) { /* the second stream joins once the reset is done */
@@ -336,7 +337,8 @@ bool artfHoleBatchStats(const ArtfHoleBatch* b, ArtfHoleBatchStats* out) {
     out->views = b->count, out->rays = b->rays, out->schedule = b->schedule;
     if (!b->drawn) return true;
     out->batchMs = b->batchMs;
-    for (int i = 0; i < ARTF_HIP_COUNTER_SLOTS; ++i) out->steps += b->countersHost[i], out->laneSlots += b->countersHost[ARTF_HIP_COUNTER_SLOTS + i];
+    for (int i = 0; !b->tracing.uncounted && i < ARTF_HIP_COUNTER_SLOTS; ++i)
+        out->steps += b->countersHost[i], out->laneSlots += b->countersHost[ARTF_HIP_COUNTER_SLOTS + i];
     for (int v = 0; v < b->count; ++v) out->traceMs = largest(out->traceMs, stampMs(b, v));
     return true;
 }

@@ -307,11 +307,12 @@ class Renderer:
             pass
 
 
-def tracing(precision="fp32", disc="analytic", emissivity=3.0, max_steps=0, escape_radius=0.0, jitter=0, bloom_levels=0):
+def tracing(precision="fp32", disc="analytic", emissivity=3.0, max_steps=0, escape_radius=0.0, jitter=0, bloom_levels=0, count_steps=True):
     t = lib().artfHoleTracingDefaults()
     t.precision = abi.PRECISION[precision] if isinstance(precision, str) else precision
     t.disc = abi.DISC[disc] if isinstance(disc, str) else disc
     t.emissivity, t.maxSteps, t.escapeRadius, t.jitter, t.bloomLevels = emissivity, max_steps, escape_radius, jitter, bloom_levels
+    t.uncounted = 0 if count_steps else 1
     return t
 
 

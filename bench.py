@@ -278,15 +278,21 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=No
     """The batch workload: 64 views (seed 4) of 128..1024 pixels a side, spin U[0,.998], inclination
     U[5,85], eye r U[20,100], disc outer U[10,40], each with its own rings and fluid; rays of up to 5000
     steps; one launch replayed from a HIP graph, schedule from the tuner (a workgroup a tile, or the
-    persistent queued kernel with its workgroups, chunk, refill and check interval). Beside it, the
+    persistent queued kernel with its workgroups, chunk, refill and check interval), its steps not counted
+    (a counted batch of the same views gives the step counts). Beside it, the
     analytic disc through the same batch and through the tile kernel one launch per view."""
     from artful_amd.workloads import uneven_views
     views, _ = uneven_views(count, seed)
     out = {"views": count, "seed": seed, "max_steps": max_steps}
     for disc in ("painted", "analytic"):
         trc = native.tracing(precision="fp32", disc=disc, max_steps=max_steps)
+        # the timed batch does not count its steps (as the timed frames); a counted one gives the counts
+        counted = native.HoleBatch(views, trc)
+        counted.draw()
+        cst = counted.stats()
+        counted.close()
         t0 = time.perf_counter()
-        b = native.HoleBatch(views, trc)
+        b = native.HoleBatch(views, native.tracing(precision="fp32", disc=disc, max_steps=max_steps, count_steps=False))
         setup = time.perf_counter() - t0
 
         def run(n):
@@ -299,7 +305,7 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=No
             return st, float(np.median(ms)), float(np.median(trace))
 
         b.draw()
-        st0, default_ms, _ = run(draws)
+        _, default_ms, _ = run(draws)
         t0 = time.perf_counter()
         if schedule:
             b.set_schedule(*schedule)
@@ -310,10 +316,9 @@ def batched_views(native, count=64, seed=4, max_steps=5000, draws=5, schedule=No
         st, batch_ms, trace_ms = run(draws)
         lat = b.latency()
         b.close()
-        leg = {"setup_s": setup, "rays": int(st.rays), "steps_per_ray": st.steps / st.rays,
+        leg = {"setup_s": setup, "rays": int(st.rays), "steps_per_ray": cst.steps / cst.rays,
                "graph_ms": batch_ms, "trace_ms": trace_ms, "rays_per_s": st.rays / (batch_ms * 1e-3),
-               "active_lane_fraction": st.steps / max(st.laneSlots, 1),
-               "default_schedule_ms": default_ms, "default_active_lane_fraction": st0.steps / max(st0.laneSlots, 1),
+               "active_lane_fraction": cst.steps / max(cst.laneSlots, 1), "default_schedule_ms": default_ms,
                "schedule": {"workgroups": sched.workgroups, "chunk_tiles": sched.chunkTiles, "refill": sched.refill,
                             "steps_per_check": sched.stepsPerCheck}, "tune_s": tune_s,
                "latency_ms": {"min": float(lat.min()), "median": float(np.median(lat)), "max": float(lat.max()),

@@ -33,7 +33,8 @@ typedef struct ArtfHoleTracing {
     int bloomLevels;    /* 0: the reference's rule (while both sides exceed 3, up to 8) */
     /* 0: each frame's Nystrom steps counted into ArtfHoleStats (steps, laneSlots); 1: not counted (those
      * read 0), and the trace skips the two wave reductions and two atomics a wave the counts take, 1.5% of
-     * a 1080p trace. The renderer's frames only: a batch (artful/batch.h) always counts. */
+     * a 1080p trace. A batch (artful/batch.h) made with it counts nothing either (its stats' steps and
+     * laneSlots read 0). */
     int uncounted;
 } ArtfHoleTracing;
 
