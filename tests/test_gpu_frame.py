@@ -211,6 +211,47 @@ def test_renderer_frames_back_to_back(name, w, h, gpu, scene_of):
         sc.close()
 
 
+def test_uncounted_frames_are_the_same_frames(gpu):
+    """ArtfHoleTracing.uncounted: frames queued back to back without step counts give the same pixels and
+    HDR as counted ones, bit for bit, with steps and lane-slots 0; counted again, the counts come back
+    (and the counted frame after uncounted ones counts only its own steps). The same for a batch."""
+    from tests.golden import cases
+    desc, cam = cases.make(cases.CASES["config1"])
+    s = gpu.Scene(desc, cam)
+    r = gpu.Renderer(160, 90)
+    r.set_tracing("fp32")
+    assert r.draw(s)
+    want = (r.read_pixels(), r.read_hdr())
+    st = r.hole_stats()
+    counts = (st.steps, st.laneSlots)
+    assert counts[0] > 0 and counts[0] <= counts[1]
+    r.set_tracing("fp32", count_steps=False)
+    for _ in range(3):
+        assert r.draw(s)
+    st = r.hole_stats()
+    assert (st.steps, st.laneSlots) == (0, 0) and st.frames == 3
+    np.testing.assert_array_equal(r.read_hdr().view(np.uint32), want[1].view(np.uint32))
+    np.testing.assert_array_equal(r.read_pixels(), want[0])
+    r.set_tracing("fp32")
+    assert r.draw(s)
+    st = r.hole_stats()
+    assert (st.steps, st.laneSlots) == counts
+    r.close()
+    s.close()
+    from artful_amd.workloads import uneven_views
+    views, _ = uneven_views(count=3, seed=4, sizes=(64, 96))
+    images = {}
+    for count in (True, False):
+        b = gpu.HoleBatch(views, gpu.tracing(precision="fp32", disc="painted", count_steps=count))
+        b.draw()
+        bs = b.stats()
+        images[count] = [b.read_hdr(v) for v in range(len(views))]
+        assert (bs.steps > 0) == count and (bs.laneSlots > 0) == count
+        b.close()
+    for a, c in zip(images[True], images[False]):
+        np.testing.assert_array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
 def test_renderer_resizes_and_precisions_between_queued_frames(gpu, scene_of):
     """Frames queued back to back across resizes (buffers replaced while frames may be in flight) and
     precision changes: each frame read back equals a fresh renderer's frame of the same size and
